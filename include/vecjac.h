@@ -49,6 +49,28 @@ vj_fint sbjac2_(const vj_fint *n, const float *a11, const float *a22, const floa
 vj_fint cbjac2_(const vj_fint *n, const float *a11, const float *a22, const float *a21r, const float *a21i,
                 float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p);
 
+/* One-sided Jacobi SVD of the m x n matrix G (m >= n), replacing dvjsvd_
+ * (dvjsvd.h:10, dvjsvd.c:40).  On return G holds U (unit columns), V the
+ * right singular vectors, and sigma_j = fS[j] * 2^eS[j].  js is the stp x n
+ * step table (every row a permutation of 0..n-1, else -9), swp the sweep
+ * limit, iwork[0] == 0 means V := I on entry.  work is not used (the
+ * reference's JTRACE file name is not supported).  Returns the number of
+ * sweeps, or a negative code with the reference's meaning. */
+vj_fint dvjsvd_(const vj_fnat *m, const vj_fnat *n, double *G, const vj_fnat *ldG, double *V, const vj_fnat *ldV,
+                double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work,
+                unsigned *iwork);
+/* Complex (split re/im) twin; replaces zvjsvd_ (zvjsvd.h:10, zvjsvd.c:40). */
+vj_fint zvjsvd_(const vj_fnat *m, const vj_fnat *n, double *Gr, const vj_fnat *ldGr, double *Gi, const vj_fnat *ldGi,
+                double *Vr, const vj_fnat *ldVr, double *Vi, const vj_fnat *ldVi, double *eS, double *fS,
+                const unsigned *js, const unsigned *stp, const unsigned *swp, double *work, unsigned *iwork);
+/* FP32 twins; replace svjsvd_ (svjsvd.h:10) and cvjsvd_ (cvjsvd.h:10). */
+vj_fint svjsvd_(const vj_fnat *m, const vj_fnat *n, float *G, const vj_fnat *ldG, float *V, const vj_fnat *ldV,
+                float *eS, float *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, float *work,
+                unsigned *iwork);
+vj_fint cvjsvd_(const vj_fnat *m, const vj_fnat *n, float *Gr, const vj_fnat *ldGr, float *Gi, const vj_fnat *ldGi,
+                float *Vr, const vj_fnat *ldVr, float *Vi, const vj_fnat *ldVi, float *eS, float *fS,
+                const unsigned *js, const unsigned *stp, const unsigned *swp, float *work, unsigned *iwork);
+
 /* Parallel Jacobi strategy table; replaces pjs() (pjs.h:18, pjs.c:7).
  * id 2 (ME): n-1 steps; id 4 (MM): n steps; n even.  Returns a malloc'd
  * stp x n table (free with free() or pjs_free()), NULL on bad arguments. */
@@ -66,6 +88,24 @@ vj_fint vjh_sbjac2(vj_fint n, const float *a11, const float *a22, const float *a
                    float *c, float *at, float *l1, float *l2, unsigned *p, void *stream);
 vj_fint vjh_cbjac2(vj_fint n, const float *a11, const float *a22, const float *a21r, const float *a21i,
                    float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream);
+
+/* Plain SVD on device memory: same driver as ?vjsvd_, with relaxed shape
+ * rules (any even n <= m, any ld >= m, no alignment rule) and a device step
+ * table js (validated on the device, -9 / -13 when a row is not a
+ * permutation).  The complex forms take one leading dimension for both
+ * planes.  vpreset != 0 is the reference's iwork[0] != 0.  stats, if not
+ * NULL, receives 66 words: sweeps, sweeps launched, and the number of
+ * transformations in each of the first 64 sweeps.  Synchronises `stream`. */
+vj_fint vjh_dvjsvd(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS, double *fS,
+                   const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats);
+vj_fint vjh_zvjsvd(vj_fnat m, vj_fnat n, double *Gr, double *Gi, vj_fnat ldG, double *Vr, double *Vi, vj_fnat ldV,
+                   double *eS, double *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
+                   void *stream, unsigned *stats);
+vj_fint vjh_svjsvd(vj_fnat m, vj_fnat n, float *G, vj_fnat ldG, float *V, vj_fnat ldV, float *eS, float *fS,
+                   const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats);
+vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, float *Vr, float *Vi, vj_fnat ldV,
+                   float *eS, float *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
+                   void *stream, unsigned *stats);
 
 /* Message of the last failure on the calling thread. */
 const char *vjh_last_error(void);

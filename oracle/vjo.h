@@ -57,6 +57,11 @@ void vjo_cdpscl(vjo_fnat m, const float *xr, const float *xi, const float *yr, c
 float vjo_sjrot(vjo_fint n, float *x, float *y, float c, float at);
 float vjo_sgsscl(vjo_fint m, float t, float *x, float *y, const float *e, const float *f);
 
+/* column-norm definition used by the drivers: 0 = the reference's long-double
+ * dnormf_ (default), 1 = the GPU's correctly rounded scaled norm */
+void vjo_set_norm_mode(int mode);
+int vjo_get_norm_mode(void);
+
 /* SVD drivers: reference semantics (dvjsvd.h:10, zvjsvd.h:10, svjsvd.h:10, cvjsvd.h:10) */
 vjo_fint vjo_dvjsvd(const vjo_fnat *m, const vjo_fnat *n, double *G, const vjo_fnat *ldG, double *V, const vjo_fnat *ldV, double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work, unsigned *iwork);
 vjo_fint vjo_zvjsvd(const vjo_fnat *m, const vjo_fnat *n, double *Gr, const vjo_fnat *ldGr, double *Gi, const vjo_fnat *ldGi, double *Vr, const vjo_fnat *ldVr, double *Vi, const vjo_fnat *ldVi, double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work, unsigned *iwork);

@@ -20,6 +20,11 @@ _fnat = C.c_uint64
 _vp = C.c_void_p
 
 
+class Cplx(C.Structure):
+    """double complex return value (two SSE-class doubles: xmm0, xmm1)."""
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [("sweeps", C.c_uint), ("sT", C.c_int), ("sV", C.c_int), ("rescales", C.c_uint),
                 ("swt", C.c_uint64 * 64)]
@@ -65,6 +70,8 @@ def oracle() -> C.CDLL:
         L.vjo_zgsscl.restype = C.c_double
         L.vjo_zgsscl.argtypes = [_fint, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]
         L.vjo_sdpscl.restype, L.vjo_sdpscl.argtypes = C.c_float, [_fnat, _vp, _vp, _vp, _vp]
+        L.vjo_set_norm_mode.restype, L.vjo_set_norm_mode.argtypes = None, [C.c_int]
+        L.vjo_get_norm_mode.restype, L.vjo_get_norm_mode.argtypes = C.c_int, []
         for nm, t in (("vjo_dvjsvd_batch", C.c_double), ("vjo_svjsvd_batch", C.c_float)):
             f = getattr(L, nm)
             f.restype = C.c_int64
@@ -87,7 +94,7 @@ def ref() -> C.CDLL | None:
             f.restype, f.argtypes = _fint, [P(_fint)] + [_vp] * 10
         L.ddpscl_.restype, L.ddpscl_.argtypes = C.c_double, [P(_fnat), _vp, _vp, _vp, _vp]
         L.sdpscl_.restype, L.sdpscl_.argtypes = C.c_float, [P(_fnat), _vp, _vp, _vp, _vp]
-        L.zdpscl_.restype, L.zdpscl_.argtypes = C.c_double * 2, [P(_fnat), _vp, _vp, _vp, _vp, _vp, _vp]
+        L.zdpscl_.restype, L.zdpscl_.argtypes = Cplx, [P(_fnat), _vp, _vp, _vp, _vp, _vp, _vp]
         L.dnormf_.restype, L.dnormf_.argtypes = C.c_double, [P(_fnat), _vp, _vp, _vp, _vp, _vp]
         L.djrot_.restype, L.djrot_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp]
         L.djrotf_.restype, L.djrotf_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp]
@@ -168,6 +175,25 @@ def vjsvd_core(kind: str, G, V=None, js=None, stp=0, swp=999, strict=False, Gi=N
            _ptr(eS), _ptr(fS), _ptr(js), C.byref(C.c_uint(stp)), C.byref(C.c_uint(swp)),
            _ptr(work), _ptr(iw), C.byref(st))
     return rc, eS, fS, V, Vi, st
+
+
+class norm_mode:
+    """Context manager selecting the oracle's column-norm definition:
+    0 = the reference's long-double dnormf_, 1 = the GPU's correctly rounded
+    scaled norm."""
+
+    def __init__(self, mode: int):
+        self.mode = mode
+
+    def __enter__(self):
+        L = oracle()
+        self.prev = L.vjo_get_norm_mode()
+        L.vjo_set_norm_mode(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        oracle().vjo_set_norm_mode(self.prev)
+        return False
 
 
 def sigma(eS: np.ndarray, fS: np.ndarray) -> np.ndarray:

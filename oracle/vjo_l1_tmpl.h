@@ -6,11 +6,67 @@
  * VL here (lanes past the end are simply not updated).
  */
 
+/* The GPU's column norm, restated for comparison runs (vjo_norm_mode 1):
+ * scale by 2^-k, k the exponent of max|x_i| over re and im, accumulate the
+ * squares exactly enough (double-double for FP64, double for FP32 data),
+ * take one corrected square root and return (e,f) = (k + e(r), f(r)).  The
+ * result is the correctly rounded norm except within a few ulp^2 of a
+ * rounding tie, so it matches the GPU whatever the summation order. */
+static T SFX(norm_cr)(const vjo_fnat m, const T *const xr, const T *const xi, T *const e0, T *const f0)
+{
+  T mx = (T)0;
+  for (vjo_fnat i = 0u; i < m; ++i) {
+    mx = SFX(xmax)(mx, SFX(abs)(xr[i]));
+    if (xi)
+      mx = SFX(xmax)(mx, SFX(abs)(xi[i]));
+  }
+  if (!(mx > (T)0)) {
+    *e0 = -(T)HUGE_VAL;
+    *f0 = (T)1;
+    return (T)0;
+  }
+  if (!(mx <= (T)TMAX)) {
+    *e0 = (T)HUGE_VAL;
+    *f0 = (T)1;
+    return mx;
+  }
+  const int k = ilogb((double)mx);
+  double s = 0.0, c = 0.0;
+  for (vjo_fnat i = 0u; i < m; ++i) {
+    for (int part = 0; part < (xi ? 2 : 1); ++part) {
+      const double a = ldexp((double)(part ? xi[i] : xr[i]), -k);
+      if (sizeof(T) == sizeof(double)) {
+        const double p = a * a;
+        const double e = fma(a, a, -p);
+        double t;
+        two_sum_d(s, p, &s, &t);
+        c += (t + e);
+      }
+      else
+        s += a * a;
+    }
+  }
+  double hi, lo;
+  two_sum_d(s, c, &hi, &lo);
+  double r = sqrt(hi);
+  if (r > 0.0) {
+    const double d = fma(-r, r, hi) + lo;
+    r = r + d / (2.0 * r);
+  }
+  const T rt = (T)r;
+  const int q = ilogb((double)rt);
+  *e0 = (T)(k + q);
+  *f0 = (T)ldexp((double)rt, -q);
+  return (T)ldexp((double)rt, k);
+}
+
 /* ?nrm2 with (e,f) split: restates dnormf_ (dnormf.c:5-16, the reference's
  * long-double norm) as the column norm; dnorm2_ (dnorm2.c:9-14) delegates
  * the same quantity to a BLAS ?nrm2 that is not part of the reference. */
 T RN(norm2)(const vjo_fnat m, const T *const x, T *const e0, T *const f0)
 {
+  if (vjo_norm_mode)
+    return SFX(norm_cr)(m, x, (const T*)NULL, e0, f0);
   long double sq = 0.0L;
   for (vjo_fnat i = 0u; i < m; ++i) {
     const long double l = x[i];
@@ -26,6 +82,8 @@ T RN(norm2)(const vjo_fnat m, const T *const x, T *const e0, T *const f0)
  * quantity, the rounding path is this oracle's own) */
 T CN(norm2)(const vjo_fnat m, const T *const xr, const T *const xi, T *const e0, T *const f0)
 {
+  if (vjo_norm_mode)
+    return SFX(norm_cr)(m, xr, xi, e0, f0);
   long double sq = 0.0L;
   for (vjo_fnat i = 0u; i < m; ++i) {
     const long double lr = xr[i];

@@ -32,6 +32,14 @@ def lib() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m vecjac_amd.build`")
+    # One HIP runtime per process: when torch is importable its bundled
+    # libamdhip64.so.7 is loaded first, and the library's own dependency on
+    # that soname then binds to the same copy (device memory and streams are
+    # shared with torch instead of being split over two runtimes).
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     L = C.CDLL(str(LIB_PATH))
     P = C.POINTER
     # reference-compatible 2x2 EVD (host arrays)
@@ -60,8 +68,12 @@ def _declare_svd(L) -> None:
     cv = [P(_fnat), P(_fnat), _pd, P(_fnat), _pd, P(_fnat), _pd, P(_fnat), _pd, P(_fnat), _pd, _pd, _pu,
           P(C.c_uint), P(C.c_uint), _pd, _pu]
     for name, args in (("dvjsvd_", sv), ("svjsvd_", sv), ("zvjsvd_", cv), ("cvjsvd_", cv)):
-        if hasattr(L, name):
-            _decl(L, name, _fint, args)
+        _decl(L, name, _fint, args)
+    U = C.c_uint
+    dsv = [_fnat, _fnat, _pd, _fnat, _pd, _fnat, _pd, _pd, _pu, U, U, U, C.c_void_p, _pu]
+    dcv = [_fnat, _fnat, _pd, _pd, _fnat, _pd, _pd, _fnat, _pd, _pd, _pu, U, U, U, C.c_void_p, _pu]
+    for name, args in (("vjh_dvjsvd", dsv), ("vjh_svjsvd", dsv), ("vjh_zvjsvd", dcv), ("vjh_cvjsvd", dcv)):
+        _decl(L, name, _fint, args)
 
 
 def last_error() -> str:

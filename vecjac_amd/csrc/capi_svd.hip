@@ -1,0 +1,254 @@
+// C-ABI entry points of the plain one-sided Jacobi SVD.
+//   dvjsvd_/zvjsvd_/svjsvd_/cvjsvd_: the reference's prototypes (dvjsvd.h:10,
+//     zvjsvd.h:10, svjsvd.h:10, cvjsvd.h:10) on host arrays, with the
+//     reference's argument checks in the reference's order;
+//   vjh_?vjsvd: the same driver on device arrays and a device step table.
+#include "svd_common.hpp"
+#include "../../include/vecjac.h"
+
+#include <stdint.h>
+#include <string.h>
+#include <vector>
+#ifdef __x86_64__
+#include <xmmintrin.h>
+#endif
+
+using namespace vj;
+
+namespace {
+
+inline bool fpenv_bad()
+{
+#ifdef __x86_64__
+  return ((_mm_getcsr() & 0xFFC0u) != 0x1F80u); // vec.h:19-25
+#else
+  return false;
+#endif
+}
+inline bool misaligned(const void *p) { return ((uintptr_t)p & 0x3Fu) != 0u; }
+
+// Every row of the table must be a permutation of 0..n-1: the pairs of a
+// step are then disjoint and the column updates race-free (the reference
+// relies on the same invariant of its strategy tables).
+bool table_ok(const unsigned *js, unsigned stp, unsigned long long n)
+{
+  std::vector<unsigned> seen((size_t)n, 0u);
+  for (unsigned s = 0; s < stp; ++s) {
+    const unsigned *r = js + (size_t)s * n;
+    for (unsigned long long i = 0; i < n; ++i) {
+      if (r[i] >= n || seen[r[i]] == s + 1u)
+        return false;
+      seen[r[i]] = s + 1u;
+    }
+  }
+  return true;
+}
+
+__global__ void k_table_check(const unsigned *js, unsigned n, int *bad)
+{
+  extern __shared__ unsigned mark[];
+  const unsigned *r = js + (size_t)blockIdx.x * n;
+  for (unsigned i = threadIdx.x; i < (n + 31u) / 32u; i += blockDim.x)
+    mark[i] = 0u;
+  __syncthreads();
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+    const unsigned v = r[i];
+    if (v >= n || (atomicOr(&mark[v >> 5], 1u << (v & 31u)) & (1u << (v & 31u))))
+      atomicOr(bad, 1);
+  }
+}
+
+template <typename T, bool CPLX>
+int host_vjsvd(const vj_fnat *m, const vj_fnat *n, T *Gr, const vj_fnat *ldGr, T *Gi, const vj_fnat *ldGi,
+               T *Vr, const vj_fnat *ldVr, T *Vi, const vj_fnat *ldVi, T *eS, T *fS, const unsigned *js,
+               const unsigned *stp, const unsigned *swp, T *work, unsigned *iwork)
+{
+  constexpr unsigned VL = (unsigned)Lim<T>::VL;
+  const vj_fnat n_2 = (*n >> 1u);
+  if (fpenv_bad())
+    return CPLX ? -18 : -14;
+  if (!*n)
+    return 0;
+  if (*m < *n) return -1;
+  if (*m & (VL - 1u)) return -1;
+  if (*n & 1u) return -2;
+  if (n_2 & (VL - 1u)) return -2;
+  if (misaligned(Gr)) return -3;
+  if (*ldGr < *m) return -4;
+  if (*ldGr & (VL - 1u)) return -4;
+  if (CPLX) {
+    if (misaligned(Gi)) return -5;
+    if (*ldGi < *m) return -6;
+    if (*ldGi & (VL - 1u)) return -6;
+    if (misaligned(Vr)) return -7;
+    if (*ldVr < *n) return -8;
+    if (*ldVr & (VL - 1u)) return -8;
+    if (misaligned(Vi)) return -9;
+    if (*ldVi < *n) return -10;
+    if (*ldVi & (VL - 1u)) return -10;
+    if (misaligned(work)) return -16;
+  }
+  else {
+    if (misaligned(Vr)) return -5;
+    if (*ldVr < *n) return -6;
+    if (*ldVr & (VL - 1u)) return -6;
+    if (misaligned(work)) return -12;
+  }
+  if (!table_ok(js, *stp, *n))
+    return set_error(CPLX ? -13 : -9, "step table rows must be permutations of 0..n-1");
+
+  const size_t M = (size_t)*m, N = (size_t)*n;
+  hipStream_t s = lib_stream();
+  Staging sg(s);
+  PlainArgs<T> a{};
+  a.m = (long long)M;
+  a.n = (long long)N;
+  a.ldG = (long long)M;
+  a.ldV = (long long)N;
+  a.stp = *stp;
+  a.swp = *swp;
+  a.vpreset = (*iwork != 0u);
+  unsigned *djs = nullptr;
+  if (int rc = sg.alloc(&a.Gr, M * N)) return rc;
+  if (int rc = sg.alloc(&a.Vr, N * N)) return rc;
+  if (CPLX) {
+    if (int rc = sg.alloc(&a.Gi, M * N)) return rc;
+    if (int rc = sg.alloc(&a.Vi, N * N)) return rc;
+  }
+  if (int rc = sg.alloc(&a.eS, N)) return rc;
+  if (int rc = sg.alloc(&a.fS, N)) return rc;
+  if (int rc = sg.alloc(&djs, (size_t)*stp * N)) return rc;
+  a.js = djs;
+  VJ_HIP(hipMemcpy2DAsync(a.Gr, M * sizeof(T), Gr, (size_t)*ldGr * sizeof(T), M * sizeof(T), N, hipMemcpyHostToDevice, s));
+  if (CPLX)
+    VJ_HIP(hipMemcpy2DAsync(a.Gi, M * sizeof(T), Gi, (size_t)*ldGi * sizeof(T), M * sizeof(T), N, hipMemcpyHostToDevice, s));
+  if (a.vpreset) {
+    VJ_HIP(hipMemcpy2DAsync(a.Vr, N * sizeof(T), Vr, (size_t)*ldVr * sizeof(T), N * sizeof(T), N, hipMemcpyHostToDevice, s));
+    if (CPLX)
+      VJ_HIP(hipMemcpy2DAsync(a.Vi, N * sizeof(T), Vi, (size_t)*ldVi * sizeof(T), N * sizeof(T), N, hipMemcpyHostToDevice, s));
+  }
+  if (*stp)
+    VJ_HIP(hipMemcpyAsync(djs, js, (size_t)*stp * N * sizeof(unsigned), hipMemcpyHostToDevice, s));
+  const int rc = plain_svd<T, CPLX>(a, nullptr, s);
+  const int L_MG = CPLX ? 92 : 80, L_MV = CPLX ? 104 : 92;
+  if (rc <= VJ_EHIP)
+    return rc;
+  // on an input error the reference returns before touching V or eS/fS
+  const bool early = (rc == -L_MG) || (rc == -(L_MG + 2)) || (rc == -L_MV) || (rc == (CPLX ? -17 : -13));
+  if (!early) {
+    VJ_HIP(hipMemcpy2DAsync(Gr, (size_t)*ldGr * sizeof(T), a.Gr, M * sizeof(T), M * sizeof(T), N, hipMemcpyDeviceToHost, s));
+    if (CPLX)
+      VJ_HIP(hipMemcpy2DAsync(Gi, (size_t)*ldGi * sizeof(T), a.Gi, M * sizeof(T), M * sizeof(T), N, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipMemcpy2DAsync(Vr, (size_t)*ldVr * sizeof(T), a.Vr, N * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost, s));
+    if (CPLX)
+      VJ_HIP(hipMemcpy2DAsync(Vi, (size_t)*ldVi * sizeof(T), a.Vi, N * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipMemcpyAsync(eS, a.eS, N * sizeof(T), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipMemcpyAsync(fS, a.fS, N * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+  VJ_HIP(hipStreamSynchronize(s));
+  return rc;
+}
+
+template <typename T, bool CPLX>
+int dev_vjsvd(vj_fnat m, vj_fnat n, T *Gr, T *Gi, vj_fnat ldG, T *Vr, T *Vi, vj_fnat ldV, T *eS, T *fS,
+              const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats)
+{
+  if (!n)
+    return 0;
+  if (m < n) return -1;
+  if (n & 1u) return -2;
+  if (ldG < m) return -4;
+  if (ldV < n) return -6;
+  hipStream_t s = (hipStream_t)stream;
+  if (stp) {
+    int *bad = nullptr;
+    VJ_HIP(hipMallocAsync((void **)&bad, sizeof(int), s));
+    VJ_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_table_check, dim3(stp), dim3(256), ((n + 31) / 32) * sizeof(unsigned), s, js, (unsigned)n, bad);
+    int hbad = 0;
+    VJ_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipFreeAsync(bad, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    if (hbad)
+      return set_error(CPLX ? -13 : -9, "step table rows must be permutations of 0..n-1");
+  }
+  PlainArgs<T> a{};
+  a.m = (long long)m;
+  a.n = (long long)n;
+  a.Gr = Gr;
+  a.Gi = Gi;
+  a.ldG = (long long)ldG;
+  a.Vr = Vr;
+  a.Vi = Vi;
+  a.ldV = (long long)ldV;
+  a.eS = eS;
+  a.fS = fS;
+  a.js = js;
+  a.stp = stp;
+  a.swp = swp;
+  a.vpreset = (vpreset != 0u);
+  PlainRun run{};
+  const int rc = plain_svd<T, CPLX>(a, &run, s);
+  if (stats && rc > VJ_EHIP) {
+    // stats[0] = sweeps, [1] = launched sweeps, [2..65] = transformations per sweep
+    stats[0] = run.sweeps;
+    stats[1] = run.launched_sweeps;
+    for (int i = 0; i < 64; ++i)
+      stats[2 + i] = (unsigned)run.hist[i];
+  }
+  return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+vj_fint dvjsvd_(const vj_fnat *m, const vj_fnat *n, double *G, const vj_fnat *ldG, double *V, const vj_fnat *ldV,
+                double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work,
+                unsigned *iwork)
+{
+  return host_vjsvd<double, false>(m, n, G, ldG, nullptr, ldG, V, ldV, nullptr, ldV, eS, fS, js, stp, swp, work, iwork);
+}
+vj_fint svjsvd_(const vj_fnat *m, const vj_fnat *n, float *G, const vj_fnat *ldG, float *V, const vj_fnat *ldV,
+                float *eS, float *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, float *work,
+                unsigned *iwork)
+{
+  return host_vjsvd<float, false>(m, n, G, ldG, nullptr, ldG, V, ldV, nullptr, ldV, eS, fS, js, stp, swp, work, iwork);
+}
+vj_fint zvjsvd_(const vj_fnat *m, const vj_fnat *n, double *Gr, const vj_fnat *ldGr, double *Gi, const vj_fnat *ldGi,
+                double *Vr, const vj_fnat *ldVr, double *Vi, const vj_fnat *ldVi, double *eS, double *fS,
+                const unsigned *js, const unsigned *stp, const unsigned *swp, double *work, unsigned *iwork)
+{
+  return host_vjsvd<double, true>(m, n, Gr, ldGr, Gi, ldGi, Vr, ldVr, Vi, ldVi, eS, fS, js, stp, swp, work, iwork);
+}
+vj_fint cvjsvd_(const vj_fnat *m, const vj_fnat *n, float *Gr, const vj_fnat *ldGr, float *Gi, const vj_fnat *ldGi,
+                float *Vr, const vj_fnat *ldVr, float *Vi, const vj_fnat *ldVi, float *eS, float *fS,
+                const unsigned *js, const unsigned *stp, const unsigned *swp, float *work, unsigned *iwork)
+{
+  return host_vjsvd<float, true>(m, n, Gr, ldGr, Gi, ldGi, Vr, ldVr, Vi, ldVi, eS, fS, js, stp, swp, work, iwork);
+}
+
+vj_fint vjh_dvjsvd(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS, double *fS,
+                   const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats)
+{
+  return dev_vjsvd<double, false>(m, n, G, nullptr, ldG, V, nullptr, ldV, eS, fS, js, stp, swp, vpreset, stream, stats);
+}
+vj_fint vjh_svjsvd(vj_fnat m, vj_fnat n, float *G, vj_fnat ldG, float *V, vj_fnat ldV, float *eS, float *fS,
+                   const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats)
+{
+  return dev_vjsvd<float, false>(m, n, G, nullptr, ldG, V, nullptr, ldV, eS, fS, js, stp, swp, vpreset, stream, stats);
+}
+vj_fint vjh_zvjsvd(vj_fnat m, vj_fnat n, double *Gr, double *Gi, vj_fnat ldG, double *Vr, double *Vi, vj_fnat ldV,
+                   double *eS, double *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
+                   void *stream, unsigned *stats)
+{
+  return dev_vjsvd<double, true>(m, n, Gr, Gi, ldG, Vr, Vi, ldV, eS, fS, js, stp, swp, vpreset, stream, stats);
+}
+vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, float *Vr, float *Vi, vj_fnat ldV,
+                   float *eS, float *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
+                   void *stream, unsigned *stats)
+{
+  return dev_vjsvd<float, true>(m, n, Gr, Gi, ldG, Vr, Vi, ldV, eS, fS, js, stp, swp, vpreset, stream, stats);
+}
+
+} // extern "C"

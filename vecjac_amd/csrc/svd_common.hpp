@@ -1,0 +1,63 @@
+// Shared declarations of the SVD drivers (plain and batched).
+#pragma once
+
+#include "capi_common.hpp"
+#include "svd_pair.hpp"
+
+#include <cmath>
+
+namespace vj {
+
+// Arguments of one plain SVD on device memory (column-major, split re/im
+// planes for the complex drivers; Gi/Vi are null for real data).
+template <typename T> struct PlainArgs {
+  long long m, n;
+  T *Gr, *Gi;
+  long long ldG;
+  T *Vr, *Vi;
+  long long ldV;
+  T *eS, *fS;
+  const unsigned *js; // device step table, stp rows of n entries
+  unsigned stp, swp;
+  int vpreset;        // iwork[0] != 0: V holds a starting matrix
+};
+
+// The first two words are copied to the host once per sweep.
+struct PlainFlags {
+  int done;
+  int status;
+};
+
+// Device-resident control block of one plain SVD.
+template <typename T> struct PlainState {
+  int done;                      // converged, failed or out of sweeps
+  int status;                    // 0 or the reference's negative code
+  int zero;                      // G == 0: early return with 0 sweeps
+  int ef_init;                   // inputs accepted: V/eS/fS initialised
+  T MG, MV;                      // running max-|.| bounds (dvjsvd.c:78, 88)
+  unsigned long long MG_u, MV_u; // initial max-|.| (atomicMax on bit patterns)
+  unsigned long long nMG_u, nMV_u; // per-step maxima of updated elements
+  int NRM_EXP;                   // exponent of max/(2m) (dvjsvd.c:120-123)
+  int sT, sV;                    // accumulated power-of-two shifts
+  int sG_init, sV_init;          // initial scalings
+  int sG_next, sV_next;          // rescaling to apply during the next step
+  unsigned sweeps;               // completed sweeps with transformations
+  unsigned blocks_done;          // last-workgroup detection
+  unsigned long long swt;        // transformations in the current sweep
+  unsigned long long hist[64];   // transformations per sweep (trace)
+};
+
+struct PlainRun {
+  unsigned sweeps, launched_sweeps;
+  int sT, sV;
+  unsigned long long hist[64];
+};
+
+template <typename T, bool CPLX> int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s);
+
+// Host-side thresholds, evaluated exactly as the reference does
+// (dvjsvd.c:183-184): tol = sqrt(m) * eps/2, gst = tol * 2^FIN_EXP.
+template <typename T> inline T host_tol(long long m) { return std::sqrt((T)m) * std::ldexp(Lim<T>::EPS, -1); }
+template <typename T> inline T host_gst(T tol) { return std::ldexp(tol, (int)Lim<T>::FIN_EXP); }
+
+} // namespace vj

@@ -1,0 +1,490 @@
+// One Jacobi step on one column pair, executed by a group of B lanes (one
+// wave for B = 64, a whole workgroup for B = 256/512/1024).
+//
+// The pair's columns are held in registers (E elements per lane, row
+// lane + j*B), so a step reads each column once and writes it once:
+//   1. exact column norms in (e,f) form: max-|.| exponent scaling, then a
+//      double-double sum of squares and a corrected square root
+//      (replaces dnorm2_/znorm2_, dnorm2.c:9-14, znorm2.c:9-15);
+//   2. the scaled, compensated dot product (replaces ddpscl_/zdpscl_,
+//      ddpscl.c:7-75, zdpscl.c:21-116): TwoProd + TwoSum per element and a
+//      double-double tree reduction;
+//   3. the convergence test, Gram prescaling and action code
+//      (dvjsvd.c:349-459) and the 2x2 EVD (dbjac2i cosine form);
+//   4. the in-place update: rotation (djrot_/djrotf_/zjrot_/zjrotf_), swap
+//      (dswp_) or Gram-Schmidt correction (dgsscl_/zgsscl_), with the same
+//      fused multiply-add sequences as the reference, and the max-|.|
+//      tracking that drives the overflow control.
+// Every reduction is a butterfly (all lanes end with identical values), so
+// all decisions are group-uniform.
+#pragma once
+
+#include "vj_dev.hpp"
+
+namespace vj {
+
+// -------------------------------------------------------------- reductions
+template <typename T> __device__ __forceinline__ T shfl_xor_(T v, int m) { return __shfl_xor(v, m, 64); }
+
+template <typename T> struct DD {
+  T s, c;
+};
+
+template <typename T> __device__ __forceinline__ DD<T> dd_add(DD<T> a, DD<T> b)
+{
+  DD<T> r;
+  T t;
+  two_sum(a.s, b.s, r.s, t);
+  r.c = (a.c + b.c) + t;
+  return r;
+}
+
+// s := s + p exactly kept in (s, c)
+template <typename T> __device__ __forceinline__ void dd_acc(DD<T> &a, T p, T e)
+{
+  T t;
+  two_sum(a.s, p, a.s, t);
+  a.c += (t + e);
+}
+
+template <typename T, int B>
+__device__ __forceinline__ T red_max(T v, double *lds)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    v = xmax(v, shfl_xor_(v, o));
+  if constexpr (B > 64) {
+    constexpr int W = B / 64;
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+      lds[w] = (double)v;
+    __syncthreads();
+    T r = (T)lds[0];
+#pragma unroll
+    for (int k = 1; k < W; ++k)
+      r = xmax(r, (T)lds[k]);
+    v = r;
+  }
+  return v;
+}
+
+template <typename T, int B>
+__device__ __forceinline__ DD<T> red_dd(DD<T> v, double *lds)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    DD<T> u;
+    u.s = shfl_xor_(v.s, o);
+    u.c = shfl_xor_(v.c, o);
+    v = dd_add(v, u); // commutative and exact-error: partners agree bitwise
+  }
+  if constexpr (B > 64) {
+    constexpr int W = B / 64;
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+      lds[2 * w] = (double)v.s;
+      lds[2 * w + 1] = (double)v.c;
+    }
+    __syncthreads();
+    DD<T> r{(T)lds[0], (T)lds[1]};
+#pragma unroll
+    for (int k = 1; k < W; ++k)
+      r = dd_add(r, DD<T>{(T)lds[2 * k], (T)lds[2 * k + 1]});
+    v = r;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------- (e,f) norms
+// Accumulator type of the norm and dot reductions: double-double for FP64
+// data, plain double for FP32 data (FP32 products are exact in double).
+template <typename T> struct Acc;
+template <> struct Acc<double> { typedef double A; static constexpr bool DDACC = true; };
+template <> struct Acc<float> { typedef double A; static constexpr bool DDACC = false; };
+
+// sqrt of a double-double, rounded once (one Newton correction on the
+// residual computed with an FMA)
+__device__ __forceinline__ double dd_sqrt(DD<double> v)
+{
+  double hi, lo;
+  two_sum(v.s, v.c, hi, lo);
+  const double r = sqrt_(hi);
+  if (!(r > 0.0))
+    return r;
+  const double d = fma_(-r, r, hi) + lo;
+  return r + d / (2.0 * r);
+}
+
+// (e,f) of ||x|| = r * 2^k with r the scaled norm (r > 0, finite)
+template <typename T> __device__ __forceinline__ void ef_from(double r, int k, T &e, T &f)
+{
+  if (!(r > 0.0)) {
+    e = -inf_<T>();
+    f = (T)1;
+    return;
+  }
+  const T rt = (T)r;
+  const int q = frexp_exp(rt) - 1;
+  e = (T)(k + q);
+  f = ldexp_(rt, -q);
+}
+
+// ----------------------------------------------------------- pair context
+template <typename T> struct Step {
+  T tol, gst;        // sqrt(m) eps/2 and tol * 2^FIN_EXP (dvjsvd.c:183-184)
+  int sG, sV;        // pending power-of-two rescale of G and V in this step
+  int vtrack;        // 1: V preset (djrot_ tracking), 0: V = I (djrotf_)
+};
+
+template <typename T, bool CPLX> struct PairOut {
+  int code;          // action code (dvjsvd.c:438-453)
+  int trans;         // 1 if transformed (counted in the sweep total)
+  int err;           // 0, or the reference's negative line code
+  T tG, tV;          // max |updated G / V element| of this pair
+};
+
+// Returns the reference's action code and updates x,y (registers) in
+// place; V columns are streamed from/to global memory.  All lanes of the
+// group call this with the same p, q.
+template <typename T, bool CPLX, int B, int E>
+__device__ __forceinline__ PairOut<T, CPLX>
+pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
+          T *__restrict__ Vrp, T *__restrict__ Vrq, T *__restrict__ Vip, T *__restrict__ Viq,
+          T &eSp, T &fSp, T &eSq, T &fSq, const Step<T> &S, double *lds, const int lane)
+{
+  typedef typename Acc<T>::A A;
+  PairOut<T, CPLX> out{1, 0, 0, (T)0, (T)0};
+
+  // ---- 1. norms
+  T mx = (T)0, my = (T)0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    mx = xmax(mx, vabs(xr[j]));
+    my = xmax(my, vabs(yr[j]));
+    if constexpr (CPLX) {
+      mx = xmax(mx, vabs(xi[j]));
+      my = xmax(my, vabs(yi[j]));
+    }
+  }
+  mx = red_max<T, B>(mx, lds);
+  my = red_max<T, B>(my, lds);
+  const int kx = (mx > (T)0) ? (frexp_exp(mx) - 1) : 0;
+  const int ky = (my > (T)0) ? (frexp_exp(my) - 1) : 0;
+  DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const A a = ldexp_((A)xr[j], -kx), b = ldexp_((A)yr[j], -ky);
+    if constexpr (Acc<T>::DDACC) {
+      A p = a * a;
+      dd_acc(sx, p, fma_(a, a, -p));
+      p = b * b;
+      dd_acc(sy, p, fma_(b, b, -p));
+      p = a * b;
+      dd_acc(dr, p, fma_(a, b, -p));
+    }
+    else {
+      sx.s += a * a;
+      sy.s += b * b;
+      dr.s += a * b;
+    }
+    if constexpr (CPLX) {
+      const A c = ldexp_((A)xi[j], -kx), d = ldexp_((A)yi[j], -ky);
+      if constexpr (Acc<T>::DDACC) {
+        A p = c * c;
+        dd_acc(sx, p, fma_(c, c, -p));
+        p = d * d;
+        dd_acc(sy, p, fma_(d, d, -p));
+        // x^H y = (a - ic)(b + id) = (ab + cd) + i(ad - cb)
+        p = c * d;
+        dd_acc(dr, p, fma_(c, d, -p));
+        p = a * d;
+        dd_acc(di, p, fma_(a, d, -p));
+        p = -(c * b);
+        dd_acc(di, p, -fma_(c, b, -(c * b)));
+      }
+      else {
+        sx.s += c * c;
+        sy.s += d * d;
+        dr.s += c * d;
+        di.s += a * d - c * b;
+      }
+    }
+  }
+  if constexpr (Acc<T>::DDACC) {
+    sx = red_dd<A, B>(sx, lds);
+    sy = red_dd<A, B>(sy, lds);
+    dr = red_dd<A, B>(dr, lds);
+    if constexpr (CPLX)
+      di = red_dd<A, B>(di, lds);
+  }
+  else {
+    sx = red_dd<A, B>(sx, lds);
+    sy = red_dd<A, B>(sy, lds);
+    dr = red_dd<A, B>(dr, lds);
+    if constexpr (CPLX)
+      di = red_dd<A, B>(di, lds);
+  }
+  const double rx = (mx > (T)0) ? dd_sqrt(DD<double>{(double)sx.s, (double)sx.c}) : 0.0;
+  const double ry = (my > (T)0) ? dd_sqrt(DD<double>{(double)sy.s, (double)sy.c}) : 0.0;
+  T ex, fx, ey, fy;
+  ef_from<T>(rx, kx, ex, fx);
+  ef_from<T>(ry, ky, ey, fy);
+
+  // ---- 2. cosine a21 = y^H x / (|x||y|) of the reference's (q, p) order
+  // (dvjsvd.c:315-319: ddpscl_(m, G_q, G_p, ...)); real part symmetric,
+  // imaginary part of y^H x = -(imag of x^H y).
+  T a21r = (T)0, a21i = (T)0;
+  if ((mx > (T)0) && (my > (T)0)) {
+    const int shift = (kx + ky) - (int)(ex + ey);
+    const T de = fx * fy;
+    const T nur = ldexp_((T)((A)dr.s + (A)dr.c), shift);
+    a21r = nur / de;
+    if constexpr (CPLX) {
+      const T nui = ldexp_((T)(-((A)di.s + (A)di.c)), shift);
+      a21i = nui / de;
+    }
+  }
+  if (!isfinite_(a21r) || !isfinite_(a21i)) {
+    out.err = CPLX ? -345 : -320;
+    return out;
+  }
+
+  // ---- 3. convergence test, Gram prescaling, 2x2 EVD, action code
+  T aabs;
+  if constexpr (CPLX) {
+    const T x_ = vabs(a21r), y_ = vabs(a21i);
+    const T m_ = xmin(x_, y_), M_ = xmax(x_, y_);
+    T z = xmax(m_ / M_, (T)0);
+    z = sqrt_(fma_(z, z, (T)1));
+    aabs = z * M_;
+  }
+  else
+    aabs = vabs(a21r);
+  T c = (T)1, at = (T)0, sat = (T)0;
+  if (S.tol <= aabs) {
+    out.trans = 1;
+    const T e1 = ex, e2 = ey, f1 = fx, f2 = fy;
+    const bool ngs = eflt(e1, e2, f1, f2);
+    T maf = ngs ? f1 : f2;
+    T mae = ngs ? e1 : e2;
+    const T Maf = ngs ? f2 : f1;
+    const T Mae = ngs ? e2 : e1;
+    maf = S.gst * maf;
+    mae = mae + getexp(maf);
+    maf = getmant(maf);
+    const bool cgs = eflt(mae, Mae, maf, Maf);
+    T f12 = f1 / f2, e12 = e1 - e2, f21 = f2 / f1, e21 = e2 - e1;
+    e12 = e12 + getexp(f12);
+    f12 = getmant(f12);
+    e21 = e21 + getexp(f21);
+    f21 = getmant(f21);
+    const bool c12 = efle(e12, e21, f12, f21);
+    const T Ee = c12 ? e21 : e12;
+    const T d = xmin((T)Lim<T>::FIN_EXP - Ee, (T)0);
+    e12 = e12 + d;
+    e21 = e21 + d;
+    const T w0 = scalef(f12, e12), w1 = scalef(f21, e21);
+    T l1, l2;
+    bool P;
+    if constexpr (CPLX)
+      cjac2<T, JAC2_TAN>(w0, w1, scalef(a21r, d), scalef(a21i, d), c, at, sat, l1, l2, P);
+    else
+      rjac2<T, JAC2_TAN>(w0, w1, scalef(a21r, d), c, at, l1, l2, P);
+    out.code = (!ngs && cgs) ? 3 : ((ngs && cgs) ? -3 : (P ? -2 : 2));
+  }
+  else
+    out.code = (efcmp(ex, fx, ey, fy) < 0) ? -1 : 1;
+
+  // ---- 4. apply (dvjsvd.c:460-574)
+  T tG = (T)0, tV = (T)0;
+  const int code = out.code;
+  if ((code == 2) || (code == -2)) {
+    const bool perm = (code < 0);
+    // G pair in registers: djrot_/zjrot_ with length +-m; a complex
+    // rotation with sat == 0 is two real rotations (zjrot.c:26-35)
+    if ((at == (T)0) && (!CPLX || (sat == (T)0))) {
+      if (c == (T)1) {
+        if (perm) {
+#pragma unroll
+          for (int j = 0; j < E; ++j) {
+            T t = xr[j]; xr[j] = yr[j]; yr[j] = t;
+            if constexpr (CPLX) { t = xi[j]; xi[j] = yi[j]; yi[j] = t; }
+          }
+        }
+      }
+    }
+    else if (!CPLX || (sat == (T)0)) {
+      const bool c1 = (c == (T)1);
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        T a = perm ? fma_(-xr[j], at, yr[j]) : fma_(yr[j], at, xr[j]);
+        T b = perm ? fma_(yr[j], at, xr[j]) : fma_(-xr[j], at, yr[j]);
+        if (!c1) { a = a * c; b = b * c; }
+        xr[j] = a;
+        yr[j] = b;
+        tG = xmax(tG, xmax(vabs(a), vabs(b)));
+        if constexpr (CPLX) {
+          T a2 = perm ? fma_(-xi[j], at, yi[j]) : fma_(yi[j], at, xi[j]);
+          T b2 = perm ? fma_(yi[j], at, xi[j]) : fma_(-xi[j], at, yi[j]);
+          if (!c1) { a2 = a2 * c; b2 = b2 * c; }
+          xi[j] = a2;
+          yi[j] = b2;
+          tG = xmax(tG, xmax(vabs(a2), vabs(b2)));
+        }
+      }
+    }
+    else {
+      if constexpr (CPLX) {
+        const T cna = -at;
+        const bool c1 = (c == (T)1);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          const T x_r = xr[j], x_i = xi[j], y_r = yr[j], y_i = yi[j];
+          T ar = fma_(y_r, at, fma_(-y_i, sat, x_r));
+          T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
+          T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
+          T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
+          if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+          tG = xmax(tG, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
+          if (perm) { xr[j] = br; xi[j] = bi; yr[j] = ar; yi[j] = ai; }
+          else { xr[j] = ar; xi[j] = ai; yr[j] = br; yi[j] = bi; }
+        }
+      }
+    }
+    tG = red_max<T, B>(tG, lds);
+    // V pair, streamed: same rotation with length +-n
+    if ((at == (T)0) && (!CPLX || (sat == (T)0))) {
+      if ((c == (T)1) && perm) {
+        for (int i = lane; i < n; i += B) {
+          T t = Vrp[i]; Vrp[i] = Vrq[i]; Vrq[i] = t;
+          if constexpr (CPLX) { t = Vip[i]; Vip[i] = Viq[i]; Viq[i] = t; }
+        }
+      }
+    }
+    else {
+      const bool c1 = (c == (T)1);
+      const T cna = -at;
+      for (int i = lane; i < n; i += B) {
+        const T x_r = Vrp[i], y_r = Vrq[i];
+        if (!CPLX || (sat == (T)0)) {
+          T a = perm ? fma_(-x_r, at, y_r) : fma_(y_r, at, x_r);
+          T b = perm ? fma_(y_r, at, x_r) : fma_(-x_r, at, y_r);
+          if (!c1) { a = a * c; b = b * c; }
+          Vrp[i] = a;
+          Vrq[i] = b;
+          tV = xmax(tV, xmax(vabs(a), vabs(b)));
+          if constexpr (CPLX) {
+            const T x_i = Vip[i], y_i = Viq[i];
+            T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
+            T b2 = perm ? fma_(y_i, at, x_i) : fma_(-x_i, at, y_i);
+            if (!c1) { a2 = a2 * c; b2 = b2 * c; }
+            Vip[i] = a2;
+            Viq[i] = b2;
+            tV = xmax(tV, xmax(vabs(a2), vabs(b2)));
+          }
+        }
+        else if constexpr (CPLX) {
+          const T x_i = Vip[i], y_i = Viq[i];
+          T ar = fma_(y_r, at, fma_(-y_i, sat, x_r));
+          T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
+          T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
+          T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
+          if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+          tV = xmax(tV, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
+          if (perm) { Vrp[i] = br; Vip[i] = bi; Vrq[i] = ar; Viq[i] = ai; }
+          else { Vrp[i] = ar; Vip[i] = ai; Vrq[i] = br; Viq[i] = bi; }
+        }
+      }
+      if (S.vtrack)
+        tV = red_max<T, B>(tV, lds);
+      else
+        tV = (T)0;
+    }
+    if (!isfinite_(tG)) { out.err = CPLX ? -630 : -580; return out; }
+    if (!isfinite_(tV)) { out.err = CPLX ? -638 : -588; return out; }
+    // norms of rotated columns are recomputed at the next step
+    eSp = ex; fSp = fx; eSq = ey; fSq = fy;
+  }
+  else if ((code == 3) || (code == -3)) {
+    // dgsscl_/zgsscl_ (dgsscl.c:5-85, zgsscl.c:5-119) with t = a21
+    const bool perm = (code < 0);
+    const T x_e = -ex, y_e = -ey;
+    if (!perm) {
+      const T fy_x = fy / fx;
+      const T tfr = -a21r * fy_x;
+      const T tfi = a21i * fy_x;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const T xs = scalef_fin(xr[j], x_e);
+        const T ys = scalef_fin(yr[j], y_e);
+        if constexpr (CPLX) {
+          const T xsi = scalef_fin(xi[j], x_e), ysi = scalef_fin(yi[j], y_e);
+          const T nr = fma_(tfr, xs, fma_(-tfi, xsi, ys));
+          const T ni = fma_(tfr, xsi, fma_(tfi, xs, ysi));
+          yr[j] = scalef_fin(nr, ey);
+          yi[j] = scalef_fin(ni, ey);
+          tG = xmax(tG, xmax(vabs(yr[j]), vabs(yi[j])));
+        }
+        else {
+          yr[j] = scalef_fin(fma_(tfr, xs, ys), ey);
+          tG = xmax(tG, vabs(yr[j]));
+        }
+      }
+      eSp = ex; fSp = fx; eSq = ey; fSq = fy;
+    }
+    else {
+      const T fx_y = fx / fy;
+      const T tfr = -a21r * fx_y;
+      const T tfi = -a21i * fx_y;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const T xs = scalef_fin(xr[j], x_e);
+        const T ys = scalef_fin(yr[j], y_e);
+        if constexpr (CPLX) {
+          const T xsi = scalef_fin(xi[j], x_e), ysi = scalef_fin(yi[j], y_e);
+          const T nr = fma_(tfr, ys, fma_(-tfi, ysi, xs));
+          const T ni = fma_(tfr, ysi, fma_(tfi, ys, xsi));
+          xr[j] = yr[j];
+          xi[j] = yi[j];
+          yr[j] = scalef_fin(nr, ex);
+          yi[j] = scalef_fin(ni, ex);
+          tG = xmax(tG, xmax(vabs(yr[j]), vabs(yi[j])));
+        }
+        else {
+          xr[j] = yr[j];
+          yr[j] = scalef_fin(fma_(tfr, ys, xs), ex);
+          tG = xmax(tG, vabs(yr[j]));
+        }
+      }
+      for (int i = lane; i < n; i += B) {
+        T t = Vrp[i]; Vrp[i] = Vrq[i]; Vrq[i] = t;
+        if constexpr (CPLX) { t = Vip[i]; Vip[i] = Viq[i]; Viq[i] = t; }
+      }
+      eSp = ey; fSp = fy; eSq = ex; fSq = fx;
+    }
+    tG = red_max<T, B>(tG, lds);
+    if (!isfinite_(tG)) { out.err = CPLX ? -630 : -580; return out; }
+  }
+  else if (code == -1) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      T t = xr[j]; xr[j] = yr[j]; yr[j] = t;
+      if constexpr (CPLX) { t = xi[j]; xi[j] = yi[j]; yi[j] = t; }
+    }
+    for (int i = lane; i < n; i += B) {
+      T t = Vrp[i]; Vrp[i] = Vrq[i]; Vrq[i] = t;
+      if constexpr (CPLX) { t = Vip[i]; Vip[i] = Viq[i]; Viq[i] = t; }
+    }
+    eSp = ey; fSp = fy; eSq = ex; fSq = fx;
+  }
+  else {
+    eSp = ex; fSp = fx; eSq = ey; fSq = fy;
+  }
+  out.tG = tG;
+  out.tV = tV;
+  return out;
+}
+
+} // namespace vj

@@ -1,0 +1,421 @@
+// Plain (one matrix) one-sided Jacobi SVD on gfx950: the device side of
+// dvjsvd_/zvjsvd_/svjsvd_/cvjsvd_ (dvjsvd.c:40-638, zvjsvd.c:40-694).
+//
+// One kernel launch per step of the strategy table: workgroup k owns pair
+// (js[st][2k], js[st][2k+1]) and runs the whole step on it (norms, dot,
+// convergence test, 2x2 EVD, update of the G and V columns).  The control
+// state (max-|.| bounds, power-of-two shifts, per-sweep transformation
+// counter, convergence flag, error code) lives in device memory and is
+// updated by the last workgroup of each step, so sweeps run without any
+// host round trip; once the flag is set, later launches return at once.
+#include "svd_common.hpp"
+
+namespace vj {
+
+// ------------------------------------------------------------ setup kernels
+template <typename T>
+__global__ __launch_bounds__(256) void k_maxabs(const int m, const int n, const T *__restrict__ A, const long long ldA,
+                                                unsigned long long *__restrict__ out)
+{
+  // max |a_ij| with NaN counted as inf (dznrmx.h:10-27)
+  T v = (T)0;
+  const long long total = (long long)m * n;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+    const long long j = t / m, i = t - j * m;
+    v = xmax(xmin(vabs(A[j * ldA + i]), inf_<T>()), v);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    v = xmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(out, (unsigned long long)__double_as_longlong((double)v));
+}
+
+template <typename T, bool CPLX>
+__global__ void k_setup(const int m, PlainState<T> *st, const int vpreset)
+{
+  // dvjsvd.c:78-156 (zvjsvd.c:90-171): input checks and initial scaling
+  if (threadIdx.x || blockIdx.x)
+    return;
+  const int L_MG = CPLX ? 92 : 80, L_MV = CPLX ? 104 : 92;
+  const T MG = (T)__longlong_as_double((long long)st->MG_u);
+  const T MV = vpreset ? (T)__longlong_as_double((long long)st->MV_u) : (T)1;
+  st->sT = st->sV = 0;
+  st->sG_next = st->sV_next = 0;
+  st->sG_init = st->sV_init = 0;
+  st->sweeps = 0;
+  st->swt = 0;
+  st->nMG_u = st->nMV_u = 0ull;
+  st->blocks_done = 0u;
+  st->zero = 0;
+  st->status = 0;
+  st->done = 0;
+  st->ef_init = 0;
+  if (!(MG <= Lim<T>::MAX)) { st->status = -L_MG; st->done = 1; return; }
+  if (vpreset) {
+    if (!(MV <= Lim<T>::MAX)) { st->status = -L_MV; st->done = 1; return; }
+    if (MV <= (T)0) { st->status = CPLX ? -17 : -13; st->done = 1; return; }
+  }
+  st->ef_init = 1; // V (unless preset) and eS/fS are initialised from here on
+  if (MG == (T)0) { st->zero = 1; st->done = 1; return; }
+  const T M_m = CPLX ? (Lim<T>::MAX / ((T)(m << 2) * (T)1.41421356237309504880)) : (Lim<T>::MAX / (T)(m << 1));
+  const int NRM_EXP = frexp_exp(M_m) - 1;
+  const int eM = frexp_exp(MG) - 1;
+  const int sN = NRM_EXP - eM - 1;
+  st->NRM_EXP = NRM_EXP;
+  st->MG = ldexp_(MG, sN);
+  st->sG_init = sN;
+  st->sT = sN;
+  int sV = 0;
+  if (vpreset) {
+    const int eV = frexp_exp(MV) - 1;
+    sV = (CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R) - eV;
+    st->MV = ldexp_(MV, sV);
+  }
+  else
+    st->MV = (T)1;
+  st->sV_init = sV;
+  st->sV = sV;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_init_cols(const int m, const int n, T *__restrict__ Gr, T *__restrict__ Gi,
+                                                   const long long ldG, T *__restrict__ Vr, T *__restrict__ Vi,
+                                                   const long long ldV, T *__restrict__ eS, T *__restrict__ fS,
+                                                   const PlainState<T> *__restrict__ st, const int vpreset)
+{
+  // column j = blockIdx.x: initial power-of-two scaling of G (dscale_), of a
+  // preset V, V := I otherwise (dvjsvd.c:97-156), eS = -inf, fS = 1
+  if (!st->ef_init)
+    return;
+  const int j = blockIdx.x;
+  const int sG = st->zero ? 0 : st->sG_init;
+  const int sV = st->sV_init;
+  if (sG) {
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      Gr[j * ldG + i] = ldexp_(Gr[j * ldG + i], sG);
+      if (Gi)
+        Gi[j * ldG + i] = ldexp_(Gi[j * ldG + i], sG);
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (vpreset) {
+      if (sV) {
+        Vr[j * ldV + i] = ldexp_(Vr[j * ldV + i], sV);
+        if (Vi)
+          Vi[j * ldV + i] = ldexp_(Vi[j * ldV + i], sV);
+      }
+    }
+    else {
+      Vr[j * ldV + i] = (i == j) ? (T)1 : (T)0;
+      if (Vi)
+        Vi[j * ldV + i] = (T)0;
+    }
+  }
+  if (threadIdx.x == 0) {
+    eS[j] = -inf_<T>();
+    fS[j] = (T)1;
+  }
+}
+
+// ------------------------------------------------------------- step kernel
+template <typename T, bool CPLX, int B, int E>
+__global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restrict__ Gr, T *__restrict__ Gi,
+                                             const long long ldG, T *__restrict__ Vr, T *__restrict__ Vi,
+                                             const long long ldV, T *__restrict__ eS, T *__restrict__ fS,
+                                             const unsigned *__restrict__ row, const unsigned st_idx,
+                                             const unsigned stp, const unsigned swp, const T tol, const T gst,
+                                             const int vtrack, PlainState<T> *__restrict__ st)
+{
+  __shared__ double lds[2 * (B / 64) + 2];
+  __shared__ int last;
+  if (*(volatile int *)&st->done)
+    return;
+  const int k = blockIdx.x;
+  const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
+  const int lane = threadIdx.x;
+  Step<T> S;
+  S.tol = tol;
+  S.gst = gst;
+  S.sG = st->sG_next;
+  S.sV = st->sV_next;
+  S.vtrack = vtrack;
+
+  T *const Gp = Gr + (long long)p * ldG, *const Gq = Gr + (long long)q * ldG;
+  T *const Gpi = CPLX ? Gi + (long long)p * ldG : nullptr, *const Gqi = CPLX ? Gi + (long long)q * ldG : nullptr;
+  T xr[E], yr[E], xi[E], yi[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int i = lane + j * B;
+    const bool in = (i < m);
+    xr[j] = in ? Gp[i] : (T)0;
+    yr[j] = in ? Gq[i] : (T)0;
+    if constexpr (CPLX) {
+      xi[j] = in ? Gpi[i] : (T)0;
+      yi[j] = in ? Gqi[i] : (T)0;
+    }
+    else {
+      xi[j] = yi[j] = (T)0;
+    }
+    if (S.sG) {
+      xr[j] = ldexp_(xr[j], S.sG);
+      yr[j] = ldexp_(yr[j], S.sG);
+      if constexpr (CPLX) {
+        xi[j] = ldexp_(xi[j], S.sG);
+        yi[j] = ldexp_(yi[j], S.sG);
+      }
+    }
+  }
+  T *const Vp = Vr + (long long)p * ldV, *const Vq = Vr + (long long)q * ldV;
+  T *const Vpi = CPLX ? Vi + (long long)p * ldV : nullptr, *const Vqi = CPLX ? Vi + (long long)q * ldV : nullptr;
+  if (S.sV) {
+    for (int i = lane; i < n; i += B) {
+      Vp[i] = ldexp_(Vp[i], S.sV);
+      Vq[i] = ldexp_(Vq[i], S.sV);
+      if constexpr (CPLX) {
+        Vpi[i] = ldexp_(Vpi[i], S.sV);
+        Vqi[i] = ldexp_(Vqi[i], S.sV);
+      }
+    }
+  }
+  T eSp, fSp, eSq, fSq;
+  const PairOut<T, CPLX> o = pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, lds, lane);
+  if (!o.err && ((o.code != 1) || S.sG)) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int i = lane + j * B;
+      if (i < m) {
+        Gp[i] = xr[j];
+        Gq[i] = yr[j];
+        if constexpr (CPLX) {
+          Gpi[i] = xi[j];
+          Gqi[i] = yi[j];
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    if (o.err)
+      atomicCAS(&st->status, 0, o.err);
+    else {
+      eS[p] = eSp; fS[p] = fSp;
+      eS[q] = eSq; fS[q] = fSq;
+      atomicMax(&st->nMG_u, (unsigned long long)__double_as_longlong((double)o.tG));
+      atomicMax(&st->nMV_u, (unsigned long long)__double_as_longlong((double)o.tV));
+      if (o.trans)
+        atomicAdd(&st->swt, 1ull);
+    }
+    __threadfence();
+    last = (atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1u);
+  }
+  __syncthreads();
+  if (!last || lane)
+    return;
+  // ---- last workgroup of the step: fold the maxima, decide the next
+  // step's rescaling (dvjsvd.c:198-233) and the end of the sweep (:599-601)
+  __threadfence();
+  st->blocks_done = 0u;
+  if (st->status) {
+    st->done = 1;
+    return;
+  }
+  const T nMG = (T)__longlong_as_double((long long)st->nMG_u);
+  const T nMV = (T)__longlong_as_double((long long)st->nMV_u);
+  st->nMG_u = 0ull;
+  st->nMV_u = 0ull;
+  // st->MG already carries this step's shift (applied when it was decided)
+  T MG = st->MG;
+  MG = (nMG > MG) ? nMG : MG;
+  T MV = st->MV;
+  MV = (nMV > MV) ? nMV : MV;
+  const int rot = CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R;
+  int sG = 0, sV = 0;
+  {
+    const int eM = frexp_exp(MG) - 1;
+    if (rot - eM < 0) {
+      sG = st->NRM_EXP - eM - 1;
+      MG = ldexp_(MG, sG);
+      st->sT += sG;
+    }
+  }
+  {
+    const int eV = frexp_exp(MV) - 1;
+    if (rot - eV < 0) {
+      sV = rot - eV;
+      MV = ldexp_(MV, sV);
+      st->sV += sV;
+    }
+  }
+  st->MG = MG;
+  st->MV = MV;
+  st->sG_next = sG;
+  st->sV_next = sV;
+  if (st_idx == stp - 1u) {
+    const unsigned long long swt = st->swt;
+    if (st->sweeps < 64u)
+      st->hist[st->sweeps] = swt;
+    if (!swt)
+      st->done = 1;
+    else {
+      st->sweeps += 1u;
+      st->swt = 0ull;
+      if (st->sweeps >= swp)
+        st->done = 1;
+    }
+  }
+  __threadfence();
+}
+
+// ----------------------------------------------------------------- epilogue
+template <typename T>
+__global__ __launch_bounds__(256) void k_epilogue(const int m, const int n, T *__restrict__ Gr, T *__restrict__ Gi,
+                                                  const long long ldG, T *__restrict__ Vr, T *__restrict__ Vi,
+                                                  const long long ldV, T *__restrict__ eS, T *__restrict__ fS,
+                                                  const PlainState<T> *__restrict__ st, const unsigned swp)
+{
+  // dvjsvd.c:604-631: G_j <- G_j / fS_j * 2^-eS_j, eS_j -= sT; V *= 2^-sV
+  if (st->status || st->zero || !(st->sweeps < swp))
+    return;
+  const int j = blockIdx.x;
+  const T f_ = fS[j], s_ = -eS[j];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    Gr[j * ldG + i] = scalef(Gr[j * ldG + i] / f_, s_);
+    if (Gi)
+      Gi[j * ldG + i] = scalef(Gi[j * ldG + i] / f_, s_);
+  }
+  if (st->sV) {
+    const T sv = (T)(-st->sV);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      Vr[j * ldV + i] = scalef(Vr[j * ldV + i], sv);
+      if (Vi)
+        Vi[j * ldV + i] = scalef(Vi[j * ldV + i], sv);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    eS[j] -= (T)st->sT;
+}
+
+// ------------------------------------------------------------------ driver
+template <typename T, bool CPLX, int B, int E>
+static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, const unsigned *row, unsigned s, const T tol, const T gst, hipStream_t strm)
+{
+  hipLaunchKernelGGL((k_step<T, CPLX, B, E>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m, (int)a.n, a.Gr, a.Gi, a.ldG,
+                     a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp, tol, gst, a.vpreset, st);
+  return 0;
+}
+
+template <typename T, bool CPLX>
+using StepFn = int (*)(const PlainArgs<T> &, PlainState<T> *, const unsigned *, unsigned, T, T, hipStream_t);
+
+template <typename T, bool CPLX>
+static StepFn<T, CPLX> pick_step(long long m, int &B_out)
+{
+  // smallest group that holds the column pair in registers with <= 8
+  // elements per lane (<= 16 for the largest group)
+  if (m <= 64) { B_out = 64; return &launch_steps<T, CPLX, 64, 1>; }
+  if (m <= 128) { B_out = 64; return &launch_steps<T, CPLX, 64, 2>; }
+  if (m <= 256) { B_out = 64; return &launch_steps<T, CPLX, 64, 4>; }
+  if (m <= 512) { B_out = 64; return &launch_steps<T, CPLX, 64, 8>; }
+  if (m <= 1024) { B_out = 256; return &launch_steps<T, CPLX, 256, 4>; }
+  if (m <= 2048) { B_out = 256; return &launch_steps<T, CPLX, 256, 8>; }
+  if (m <= 4096) { B_out = 1024; return &launch_steps<T, CPLX, 1024, 4>; }
+  if (m <= 8192) { B_out = 1024; return &launch_steps<T, CPLX, 1024, 8>; }
+  if (m <= 16384) { B_out = 1024; return &launch_steps<T, CPLX, 1024, 16>; }
+  return nullptr;
+}
+
+template <typename T, bool CPLX>
+int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
+{
+  if (a.n == 0)
+    return 0;
+  int B = 0;
+  StepFn<T, CPLX> step = pick_step<T, CPLX>(a.m, B);
+  if (!step)
+    return set_error(-1, "m = %lld exceeds the largest supported column length (16384)", a.m);
+  PlainState<T> *st = nullptr;
+  VJ_HIP(hipMallocAsync((void **)&st, sizeof(PlainState<T>), s));
+  VJ_HIP(hipMemsetAsync(st, 0, sizeof(PlainState<T>), s));
+  const int mx_blocks = 1024;
+  hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gr, a.ldG, &st->MG_u);
+  if (CPLX)
+    hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gi, a.ldG, &st->MG_u);
+  if (a.vpreset) {
+    hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.n, (int)a.n, a.Vr, a.ldV, &st->MV_u);
+    if (CPLX)
+      hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.n, (int)a.n, a.Vi, a.ldV, &st->MV_u);
+  }
+  hipLaunchKernelGGL((k_setup<T, CPLX>), dim3(1), dim3(64), 0, s, (int)a.m, st, a.vpreset);
+  hipLaunchKernelGGL(k_init_cols<T>, dim3((unsigned)a.n), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gr, a.Gi, a.ldG,
+                     a.Vr, a.Vi, a.ldV, a.eS, a.fS, st, a.vpreset);
+  if (int rc = check_launch())
+    return rc;
+  const T tol = host_tol<T>(a.m);
+  const T gst = host_gst<T>(tol);
+  // Sweeps are enqueued one ahead of the convergence check: the host only
+  // reads the device flag of sweep k-1 after sweep k has been queued.
+  PlainFlags *hf = nullptr;
+  VJ_HIP(hipHostMalloc((void **)&hf, 2 * sizeof(PlainFlags), hipHostMallocDefault));
+  hipEvent_t ev[2];
+  VJ_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  VJ_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int rc = 0;
+  unsigned launched = 0;
+  for (unsigned sw = 0; sw < a.swp; ++sw) {
+    for (unsigned k = 0; k < a.stp; ++k)
+      step(a, st, a.js + (size_t)k * a.n, k, tol, gst, s);
+    if ((rc = check_launch()))
+      break;
+    ++launched;
+    if (hipMemcpyAsync(&hf[sw & 1u], &st->done, sizeof(PlainFlags), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipEventRecord(ev[sw & 1u], s) != hipSuccess) {
+      rc = set_error(VJ_EHIP, "flag copy failed");
+      break;
+    }
+    if (sw >= 1) {
+      if (hipEventSynchronize(ev[(sw - 1) & 1u]) != hipSuccess) {
+        rc = set_error(VJ_EHIP, "event sync failed");
+        break;
+      }
+      if (hf[(sw - 1) & 1u].done)
+        break;
+    }
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_epilogue<T>, dim3((unsigned)a.n), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gr, a.Gi, a.ldG,
+                       a.Vr, a.Vi, a.ldV, a.eS, a.fS, st, a.swp);
+    rc = check_launch();
+  }
+  PlainState<T> h;
+  if (!rc) {
+    if (hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+      rc = set_error(VJ_EHIP, "state copy failed");
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  (void)hipHostFree(hf);
+  (void)hipFreeAsync(st, s);
+  if (rc)
+    return rc;
+  if (run) {
+    run->sweeps = h.sweeps;
+    run->sT = h.sT;
+    run->sV = h.sV;
+    run->launched_sweeps = launched;
+    for (int i = 0; i < 64; ++i)
+      run->hist[i] = h.hist[i];
+  }
+  if (h.status)
+    return h.status;
+  if (h.zero)
+    return 0;
+  return (int)h.sweeps;
+}
+
+template int plain_svd<double, false>(const PlainArgs<double> &, PlainRun *, hipStream_t);
+template int plain_svd<double, true>(const PlainArgs<double> &, PlainRun *, hipStream_t);
+template int plain_svd<float, false>(const PlainArgs<float> &, PlainRun *, hipStream_t);
+template int plain_svd<float, true>(const PlainArgs<float> &, PlainRun *, hipStream_t);
+
+} // namespace vj
