@@ -123,7 +123,220 @@ def cpu_bjac2(kind: str, ins_np):
             "sample": f"{reps} x {nsamp} problems ({'AVX-512 reference build, OpenMP' if use_ref else 'oracle C restatement, 1 thread'}), incl. staging copies"}
 
 
+
+# --------------------------------------------------------------------------- batched SVD
+def _bsvd_alg_bytes(m, n, info, transforms, swp, w):
+    """Algorithmic HBM bytes of one batched launch: every step reads each
+    column once (w m n) and a transformed pair writes its two G columns and
+    reads/writes its two V columns (w (2m + 4n)); a converged matrix runs
+    info + 1 sweeps of n - 1 steps (the last one without transformations)."""
+    m = m.astype(np.float64)
+    n = n.astype(np.float64)
+    sweeps = np.where(info < swp, info + 1, swp).astype(np.float64)
+    steps = sweeps * (n - 1)
+    return float(np.sum(steps * w * m * n + transforms.astype(np.float64) * w * (2 * m + 4 * n)))
+
+
+def bench_bsvd(args, kind: str = "d"):
+    import torch
+    from vecjac_amd import batched as BT
+    from vecjac_amd import svd as S
+    from vecjac_amd import workloads as W
+
+    nb = args.nb
+    m, n_real = W.batch_shapes(nb, 4242, 8, 128)
+    m_, npad, offG, offV, offS, tot = BT.pack_shapes(m, n_real)
+    G64 = BT.fill_gaussian(m, npad, n_real, offG, 4243, np.float64)
+
+    def run_batch(kind, steps, warmup, timed=True):
+        dt_np = np.float64 if kind == "d" else np.float32
+        dt = torch.float64 if kind == "d" else torch.float32
+        plan = BT.BatchedSVD(m, npad, kind=kind, swp=999, waves=args.waves)
+        G0 = torch.from_numpy(G64.astype(dt_np)).cuda()
+        G = torch.empty_like(G0)
+        V = torch.empty(tot[1], dtype=dt, device="cuda")
+        eS = torch.empty(tot[2], dtype=dt, device="cuda")
+        fS = torch.empty(tot[2], dtype=dt, device="cuda")
+        info = torch.empty(nb, dtype=torch.int64, device="cuda")
+        tr = torch.empty(nb, dtype=torch.int64, device="cuda")
+        stream = torch.cuda.current_stream()
+        for _ in range(warmup):
+            G.copy_(G0)
+            plan.run(G, V, eS, fS, info, tr)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * steps)]
+        t0 = time.perf_counter()
+        for i in range(steps):
+            G.copy_(G0)
+            ev[2 * i].record(stream)
+            plan.run(G, V, eS, fS, info, tr)
+            ev[2 * i + 1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)]
+        res = dict(plan=plan, wall=wall, kernel_ms=float(np.mean(kms)), info=info.cpu().numpy(),
+                   tr=tr.cpu().numpy(), G=G, eS=eS.cpu().numpy(), fS=fS.cpu().numpy(), waves=plan.waves)
+        return res
+
+    w = 8 if kind == "d" else 4
+    r = run_batch(kind, args.steps, args.warmup)
+    info, tr = r["info"], r["tr"]
+    if np.any(info < 0):
+        raise RuntimeError(f"batched SVD failed on {int(np.sum(info < 0))} matrices")
+    alg = _bsvd_alg_bytes(m, npad, info, tr, 999, w)
+    k_s = r["kernel_ms"] / 1e3
+    achieved = alg / k_s / 1e9
+    value = nb * args.steps / r["wall"]
+    out = {
+        "metric": "SVDs/s (batched vs plain) at sigma rel err <= 8 n eps",
+        "value": value, "unit": "SVDs/s",
+        "ms_per_step": r["wall"] / args.steps * 1e3,
+        "dtype": "f64" if kind == "d" else "f32",
+        "config": {"workload": f"batched {'FP64' if kind == 'd' else 'FP32'} one-sided Jacobi SVD of {nb} matrices, "
+                               "n ~ U{8..128} (odd n padded with a zero column), m ~ U{n..4n}, N(0,1) entries, "
+                               "one persistent launch",
+                   "batch": nb, "waves_per_workgroup": r["waves"],
+                   "sweeps_mean": float(np.mean(info)), "sweeps_max": int(np.max(info))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bsvd",
+                     "kernel_avg_ms": r["kernel_ms"], "alg_bytes_per_launch": alg},
+    }
+    # sigma accuracy on a sample, against the oracle on the same matrices
+    if not args.no_check:
+        out["sigma_check"] = _bsvd_sigma_check(m, npad, offG, offS, G64, r, kind)
+    # plain per-matrix launches on a sample of the same batch
+    if not args.no_plain:
+        out["plain"] = _bsvd_plain_sample(m, npad, offG, G64, kind, args.plain_sample)
+        out["plain"]["batched_over_plain"] = value / out["plain"]["value"]
+    # FP32 run of the same batch (one timed step)
+    if kind == "d" and not args.no_fp32:
+        r32 = run_batch("s", 1, 1)
+        out["fp32"] = {"value": nb / (r32["kernel_ms"] / 1e3), "unit": "SVDs/s", "kernel_ms": r32["kernel_ms"],
+                       "sweeps_mean": float(np.mean(r32["info"]))}
+        r32["plan"].close()
+    r["plan"].close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = _bsvd_cpu(m, n_real, offG, npad, G64, kind)
+    return out
+
+
+def _bsvd_sigma_check(m, npad, offG, offS, G64, r, kind, nsamp=6):
+    from oracle import vjo
+    from vecjac_amd import svd as S
+    eps = np.finfo(np.float64 if kind == "d" else np.float32).eps
+    g = np.random.Generator(np.random.Philox(7))
+    idx = g.choice(len(m), nsamp, replace=False)
+    worst = 0.0
+    for b in idx:
+        mm, nn = int(m[b]), int(npad[b])
+        A = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm).astype(np.float64 if kind == "d" else np.float32)
+        js, stp = S.pjs(2, nn)
+        Go = vjo.aligned((nn, mm), A.dtype)
+        Go[...] = A
+        with vjo.norm_mode(1):
+            rc, eS, fS, _, _, _ = vjo.vjsvd_core(kind, Go, js=js, stp=stp, swp=999)
+        so = np.sort(S.sigma(eS, fS))
+        sg = np.sort(S.sigma(r["eS"][offS[b]: offS[b] + nn], r["fS"][offS[b]: offS[b] + nn]))
+        nz = so > 0
+        rel = float(np.max(np.abs(sg[nz] - so[nz]) / so[nz]) / (8 * nn * eps))
+        worst = max(worst, rel)
+    return {"matrices": int(nsamp), "max_rel_err_over_8n_eps": worst, "reference": "oracle, same input and table"}
+
+
+def _bsvd_plain_sample(m, npad, offG, G64, kind, nsamp):
+    import torch
+    from vecjac_amd import svd as S
+    g = np.random.Generator(np.random.Philox(8))
+    idx = g.choice(len(m), nsamp, replace=False)
+    dt_np = np.float64 if kind == "d" else np.float32
+    dt = torch.float64 if kind == "d" else torch.float32
+    mats = []
+    for b in idx:
+        mm, nn = int(m[b]), int(npad[b])
+        A = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm).astype(dt_np)
+        js, stp = S.pjs(2, nn)
+        mats.append((torch.from_numpy(A).cuda(), torch.empty((nn, nn), dtype=dt, device="cuda"),
+                     torch.empty(nn, dtype=dt, device="cuda"), torch.empty(nn, dtype=dt, device="cuda"),
+                     torch.from_numpy(js.reshape(-1).view(np.int32)).cuda(), stp))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (A, V, eS, fS, js, stp) in mats:
+        Gw = A.clone()
+        S.device_vjsvd(kind, Gw, V, eS, fS, js, stp)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    return {"value": nsamp / t, "unit": "SVDs/s", "sample": f"{nsamp} matrices of the batch, each through "
+            "vjh_dvjsvd (one launch per step)"}
+
+
+def _bsvd_cpu(m, n_real, offG, npad, G64, kind):
+    """Reference dvjsvd_ (its own sources, AVX-512; long-double norm) over a
+    thread pool, one matrix per call, inputs zero-padded to m%8 == 0 and
+    n%16 == 0 as the reference requires; the oracle port when the reference
+    build cannot run here."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import vjo
+    from vecjac_amd import svd as S
+    threads = _cpu_threads()
+    use_ref = vjo.ref() is not None and _host_has_avx512() and kind == "d"
+    g = np.random.Generator(np.random.Philox(9))
+    order = g.permutation(len(m))
+    budget = 15.0
+    if use_ref:
+        R = vjo.ref()
+        gomp = C.CDLL("libgomp.so.1")
+
+        def init():
+            gomp.omp_set_num_threads(1)
+
+        def one(b):
+            mm, nn = int(m[b]), int(n_real[b])
+            m8 = (mm + 7) // 8 * 8
+            n16 = max(16, (nn + 15) // 16 * 16)
+            m8 = max(m8, n16)
+            A = vjo.aligned((n16, m8), np.float64)
+            A[:nn, :mm] = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
+            js, stp = S.pjs(2, n16)
+            jsa = vjo.aligned(js.shape, np.uint32)
+            jsa[...] = js
+            V = vjo.aligned((n16, n16), np.float64)
+            eS = vjo.aligned(n16, np.float64)
+            fS = vjo.aligned(n16, np.float64)
+            work = vjo.aligned(8 * n16, np.float64)
+            iw = vjo.aligned(max(1, n16 // 8), np.uint32)
+            mmc, nnc = C.c_uint64(m8), C.c_uint64(n16)
+            return R.dvjsvd_(C.byref(mmc), C.byref(nnc), A.ctypes.data, C.byref(mmc), V.ctypes.data, C.byref(nnc),
+                             eS.ctypes.data, fS.ctypes.data, jsa.ctypes.data, C.byref(C.c_uint(stp)),
+                             C.byref(C.c_uint(999)), work.ctypes.data, iw.ctypes.data)
+    else:
+        def init():
+            pass
+
+        def one(b):
+            mm, nn = int(m[b]), int(npad[b])
+            A = vjo.aligned((nn, mm), np.float64 if kind == "d" else np.float32)
+            A[...] = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
+            js, stp = S.pjs(2, nn)
+            rc, *_ = vjo.vjsvd_core(kind, A, js=js, stp=stp, swp=999)
+            return rc
+    done = 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads, initializer=init) as ex:
+        while time.perf_counter() - t0 < budget and done < len(order):
+            chunk = order[done: done + 4 * threads]
+            list(ex.map(one, chunk))
+            done += len(chunk)
+    t = time.perf_counter() - t0
+    return {"value": done / t, "unit": "SVDs/s", "cores": threads, "kind": "reference" if use_ref else "port",
+            "sample": f"{done} matrices of the batch in {t:.1f} s ("
+                      + ("reference dvjsvd_ built from its sources, zero-padded to m%8, n%16, one matrix per thread"
+                         if use_ref else "oracle C restatement, one matrix per thread") + ")"}
+
+
 WORKLOADS = {
+    "bsvd": lambda a: bench_bsvd(a, "d"),
+    "bsvd32": lambda a: bench_bsvd(a, "s"),
     "bjac2": lambda a: bench_bjac2(a, "d"),
     "zbjac2": lambda a: bench_bjac2(a, "z"),
     "sbjac2": lambda a: bench_bjac2(a, "s"),
@@ -134,9 +347,15 @@ WORKLOADS = {
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="bjac2", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="bsvd", choices=sorted(WORKLOADS))
+    ap.add_argument("--nb", type=int, default=16384, help="matrices in the batched workloads")
+    ap.add_argument("--waves", type=int, default=0, help="waves per workgroup of the batched kernel (0: auto)")
+    ap.add_argument("--plain-sample", type=int, default=16, help="matrices timed through the plain path")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--lg", type=int, default=26, help="log2 batch size of the 2x2 workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()

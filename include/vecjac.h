@@ -107,6 +107,43 @@ vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, floa
                    float *eS, float *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
                    void *stream, unsigned *stats);
 
+/* ---- 3. batched SVD and SVD adjoint (no reference counterpart) --------- */
+
+/* Plan for a batch of nb independent SVDs solved in one persistent launch.
+ * kind: 0 = FP64 real, 1 = FP64 complex (split re/im), 2 = FP32 real,
+ * 3 = FP32 complex.  Host descriptors: m[b], n[b] (n even, 2 <= n <= m <=
+ * 1024), element offsets offG[b] of G_b (ld m[b]), offV[b] of V_b (ld n[b])
+ * and offS[b] of eS_b/fS_b in the packed device buffers.  waves: waves per
+ * workgroup (1, 2, 4 or 8; 0 picks from the median width).  Returns NULL on
+ * failure with *status < 0. */
+void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG,
+                       const vj_fint *offV, const vj_fint *offS, unsigned swp, int waves, vj_fint *status);
+/* One launch on device buffers: G (A in, U out; U columns of zero singular
+ * values are NaN as in the reference), V (out, starts as I), eS/fS (out),
+ * info[b] (sweeps or a negative code), transforms[b] (rotations and
+ * Gram-Schmidt steps of matrix b; may be NULL).  Gi/Vi are NULL for real
+ * kinds.  Asynchronous on `stream`; graph-capturable. */
+vj_fint vjh_bplan_run(void *plan, void *G, void *Gi, void *V, void *Vi, void *eS, void *fS, vj_fint *info,
+                      uint64_t *transforms, void *stream);
+int vjh_bplan_waves(const void *plan);
+void vjh_bplan_destroy(void *plan);
+/* One-shot FP64 batched SVD (plan, run, synchronise, destroy). */
+vj_fint vjh_dbvjsvd(vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG, const vj_fint *offV,
+                    const vj_fint *offS, double *G, double *V, double *eS, double *fS, vj_fint *info,
+                    uint64_t *transforms, unsigned swp, int waves, void *stream);
+
+/* Adjoint of the thin SVD A = U diag(s) V^T for nb matrices of m x n (m >= n,
+ * n <= 64, (2mn + 3n^2 + 2n) elements <= 64 KiB): given U (m x n), s, V
+ * (n x n) and the cotangents dU, ds, dV, writes
+ *   dA = U [diag(ds) + (F o (U^T dU - dU^T U)) S + S (F o (V^T dV - dV^T V))] V^T
+ *        + (I - U U^T) dU S^-1 V^T,   F_ij = 1 / (s_j^2 - s_i^2), F_ii = 0.
+ * Matrix b: U, dU, dA at b*m*n (ld m); V, dV at b*n*n (ld n); s, ds at b*n.
+ * Asynchronous on `stream`. */
+vj_fint vjh_dsvd_bwd(vj_fint nb, vj_fint m, vj_fint n, const double *U, const double *s, const double *V,
+                     const double *dU, const double *ds, const double *dV, double *dA, void *stream);
+vj_fint vjh_ssvd_bwd(vj_fint nb, vj_fint m, vj_fint n, const float *U, const float *s, const float *V,
+                     const float *dU, const float *ds, const float *dV, float *dA, void *stream);
+
 /* Message of the last failure on the calling thread. */
 const char *vjh_last_error(void);
 int vjh_abi_version(void);

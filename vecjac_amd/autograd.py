@@ -1,0 +1,111 @@
+"""Differentiable batched SVD: the GPU Jacobi SVD forward and the
+hand-written HIP adjoint backward, wrapped in a torch.autograd.Function.
+
+    U, S, V = jacobi_svd(A)      # A: (batch, m, n) float64 CUDA tensor, m >= n, n even
+
+The outputs are the thin SVD A = U diag(S) V^T with S sorted in decreasing
+order and each singular-vector pair normalised so that the entry of largest
+magnitude of every U column is positive (a fixed gauge, so that losses of U
+and V are well defined).  The backward applies the SVD adjoint formula with
+F_ij = 1/(s_j^2 - s_i^2) (see vjh_dsvd_bwd), which requires distinct
+singular values.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from . import batched as BT
+
+_PLANS: dict = {}
+
+
+def _plan(B: int, m: int, n: int, kind: str):
+    key = (B, m, n, kind, torch.cuda.current_device())
+    p = _PLANS.get(key)
+    if p is None:
+        p = BT.BatchedSVD([m] * B, [n] * B, kind=kind)
+        _PLANS[key] = p
+    return p
+
+
+def _forward(A: torch.Tensor):
+    if A.dim() != 3 or not A.is_cuda:
+        raise ValueError("A must be a (batch, m, n) CUDA tensor")
+    B, m, n = A.shape
+    if m < n or n % 2:
+        raise ValueError("need m >= n and n even")
+    kind = {torch.float64: "d", torch.float32: "s"}.get(A.dtype)
+    if kind is None:
+        raise TypeError("A must be float64 or float32")
+    G = A.detach().transpose(1, 2).contiguous().reshape(-1)          # columns as rows, packed
+    V = torch.empty(B * n * n, dtype=A.dtype, device=A.device)
+    eS = torch.empty(B * n, dtype=A.dtype, device=A.device)
+    fS = torch.empty(B * n, dtype=A.dtype, device=A.device)
+    info = torch.empty(B, dtype=torch.int64, device=A.device)
+    _plan(B, m, n, kind).run(G, V, eS, fS, info)
+    S = torch.ldexp(fS, torch.where(torch.isfinite(eS), eS, torch.full_like(eS, -4096)).to(torch.int32))
+    S = S.view(B, n)
+    U = G.view(B, n, m).transpose(1, 2)
+    Vm = V.view(B, n, n).transpose(1, 2)
+    order = torch.argsort(S, dim=1, descending=True)
+    S = torch.gather(S, 1, order)
+    U = torch.gather(U, 2, order.unsqueeze(1).expand(B, m, n))
+    Vm = torch.gather(Vm, 2, order.unsqueeze(1).expand(B, n, n))
+    idx = U.abs().argmax(dim=1, keepdim=True)
+    sgn = torch.sign(torch.gather(U, 1, idx))
+    sgn = torch.where(sgn == 0, torch.ones_like(sgn), sgn)
+    return (U * sgn).contiguous(), S.contiguous(), (Vm * sgn).contiguous(), info
+
+
+def svd_backward(U, S, V, gU, gS, gV):
+    """dA of the thin SVD through the HIP adjoint kernel (vjh_dsvd_bwd)."""
+    B, m, n = U.shape
+    L = _lib.lib()
+    f = {torch.float64: L.vjh_dsvd_bwd, torch.float32: L.vjh_ssvd_bwd}[U.dtype]
+    z = lambda t, shape: t if t is not None else torch.zeros(shape, dtype=U.dtype, device=U.device)
+    gU = z(gU, (B, m, n))
+    gS = z(gS, (B, n))
+    gV = z(gV, (B, n, n))
+    cm = lambda t: t.transpose(1, 2).contiguous()                     # column-major per matrix
+    Uc, Vc, gUc, gVc = cm(U), cm(V), cm(gU), cm(gV)
+    Sc, gSc = S.contiguous(), gS.contiguous()
+    dA = torch.empty((B, n, m), dtype=U.dtype, device=U.device)
+    rc = f(B, m, n, Uc.data_ptr(), Sc.data_ptr(), Vc.data_ptr(), gUc.data_ptr(), gSc.data_ptr(), gVc.data_ptr(),
+           dA.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    _lib.check(rc, "vjh_dsvd_bwd")
+    if rc < 0:
+        raise RuntimeError(f"svd backward failed ({rc}): {_lib.last_error()}")
+    return dA.transpose(1, 2)
+
+
+class JacobiSVD(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, A):
+        U, S, V, info = _forward(A)
+        ctx.save_for_backward(U, S, V)
+        ctx.mark_non_differentiable(info)
+        return U, S, V, info
+
+    @staticmethod
+    def backward(ctx, gU, gS, gV, ginfo):
+        U, S, V = ctx.saved_tensors
+        return svd_backward(U, S, V, gU, gS, gV)
+
+
+def jacobi_svd(A: torch.Tensor):
+    """(U, S, V) of a batch of matrices, differentiable (see module doc)."""
+    U, S, V, _ = JacobiSVD.apply(A)
+    return U, S, V
+
+
+def _declare():
+    L = _lib.lib()
+    import ctypes as C
+    for name in ("vjh_dsvd_bwd", "vjh_ssvd_bwd"):
+        fn = getattr(L, name)
+        fn.restype = C.c_int64
+        fn.argtypes = [C.c_int64, C.c_int64, C.c_int64] + [C.c_void_p] * 7 + [C.c_void_p]
+
+
+_declare()

@@ -1,0 +1,102 @@
+"""Batched Jacobi SVD of many uneven matrices in one persistent launch.
+
+Layout (device memory, one buffer per quantity): matrix b has its m_b x n_b
+G (A on entry, U on exit) at element offset offG[b], column-major with
+leading dimension m_b; V_b (n_b x n_b, ld n_b) at offV[b]; eS/fS (n_b each)
+at offS[b].  Widths must be even; an odd-width matrix is solved as the
+matrix with one zero column appended (its extra singular value is 0), which
+`pack_shapes` arranges.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+KINDS = {"d": 0, "z": 1, "s": 2, "c": 3}
+
+
+def _declare(L):
+    if getattr(L, "_vj_batched_declared", False):
+        return
+    i64p = C.POINTER(C.c_int64)
+    L.vjh_bplan_create.restype = C.c_void_p
+    L.vjh_bplan_create.argtypes = [C.c_int, C.c_int64, i64p, i64p, i64p, i64p, i64p, C.c_uint, C.c_int, i64p]
+    L.vjh_bplan_run.restype = C.c_int64
+    L.vjh_bplan_run.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.vjh_bplan_destroy.restype = None
+    L.vjh_bplan_destroy.argtypes = [C.c_void_p]
+    L.vjh_bplan_waves.restype = C.c_int
+    L.vjh_bplan_waves.argtypes = [C.c_void_p]
+    L._vj_batched_declared = True
+
+
+def pack_shapes(m, n):
+    """Offsets for packed storage; odd widths are padded to even.
+    Returns (m, n_padded, offG, offV, offS, totals)."""
+    m = np.asarray(m, dtype=np.int64)
+    n = np.asarray(n, dtype=np.int64)
+    npad = n + (n & 1)
+    sizeG = m * npad
+    sizeV = npad * npad
+    offG = np.concatenate([[0], np.cumsum(sizeG)[:-1]]).astype(np.int64)
+    offV = np.concatenate([[0], np.cumsum(sizeV)[:-1]]).astype(np.int64)
+    offS = np.concatenate([[0], np.cumsum(npad)[:-1]]).astype(np.int64)
+    return m, npad, offG, offV, offS, (int(sizeG.sum()), int(sizeV.sum()), int(npad.sum()))
+
+
+class BatchedSVD:
+    """A plan for one batch of shapes; `run` is one kernel launch."""
+
+    def __init__(self, m, n, kind: str = "d", swp: int = 999, waves: int = 0):
+        L = _lib.lib()
+        _declare(L)
+        self.kind = kind
+        self.m, self.n, self.offG, self.offV, self.offS, self.totals = pack_shapes(m, n)
+        if np.any(self.n != np.asarray(n)):
+            raise ValueError("odd widths must be padded first (use pack_shapes)")
+        st = C.c_int64(0)
+        P = C.POINTER(C.c_int64)
+        self._keep = [np.ascontiguousarray(a) for a in (self.m, self.n, self.offG, self.offV, self.offS)]
+        ptrs = [a.ctypes.data_as(P) for a in self._keep]
+        self.plan = L.vjh_bplan_create(KINDS[kind], len(self.m), *ptrs, swp, waves, C.byref(st))
+        if not self.plan:
+            raise RuntimeError(f"vjh_bplan_create failed ({st.value}): {_lib.last_error()}")
+        self.waves = L.vjh_bplan_waves(self.plan)
+
+    def run(self, G, V, eS, fS, info, transforms=None, Gi=None, Vi=None, stream=None) -> int:
+        import torch
+        L = _lib.lib()
+        s = (stream or torch.cuda.current_stream()).cuda_stream
+        rc = L.vjh_bplan_run(self.plan, G.data_ptr(), Gi.data_ptr() if Gi is not None else None, V.data_ptr(),
+                             Vi.data_ptr() if Vi is not None else None, eS.data_ptr(), fS.data_ptr(),
+                             info.data_ptr(), transforms.data_ptr() if transforms is not None else None, s)
+        return _lib.check(rc, "vjh_bplan_run")
+
+    def close(self):
+        if getattr(self, "plan", None):
+            _lib.lib().vjh_bplan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fill_gaussian(m, n, n_real, offG, seed: int, dtype=np.float64):
+    """Packed G with N(0,1) entries for the first n_real[b] columns of each
+    matrix and zeros in its padding column (if any)."""
+    g = np.random.Generator(np.random.Philox(seed))
+    m = np.asarray(m)
+    n = np.asarray(n)
+    total = int(np.sum(m * n))
+    G = g.standard_normal(total)  # float64 draws, so the FP32 batch is the same batch rounded
+    if G.dtype != dtype:
+        G = G.astype(dtype)
+    for b in np.nonzero(np.asarray(n_real) != n)[0]:
+        G[offG[b] + int(m[b]) * int(n_real[b]): offG[b] + int(m[b]) * int(n[b])] = 0
+    return G

@@ -1,0 +1,200 @@
+// C-ABI of the batched SVD (no reference counterpart): a plan holds the
+// shape-dependent metadata on the device (descriptors, cost order, strategy
+// tables), so that a run is a single kernel launch on resident data and can
+// be captured in a HIP graph.
+#include "svd_batched.hpp"
+#include "svd_common.hpp"
+#include "../../include/vecjac.h"
+
+#include <algorithm>
+#include <map>
+#include <numeric>
+#include <stdlib.h>
+#include <vector>
+
+using namespace vj;
+
+namespace {
+
+struct BPlan {
+  int kind = 0;            // 0 d, 1 z, 2 s, 3 c
+  int nb = 0;
+  int max_m = 0;
+  int waves = 4;
+  unsigned swp = 999u;
+  int dev = 0;
+  long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each), tab_off (nmax+1)
+  int *d_order = nullptr;
+  unsigned *d_tables = nullptr;
+  unsigned *d_queue = nullptr;
+};
+
+__global__ void k_zero_u32(unsigned *p) { *p = 0u; }
+
+int pick_waves(const std::vector<long long> &n)
+{
+  std::vector<long long> s(n);
+  std::nth_element(s.begin(), s.begin() + s.size() / 2, s.end());
+  const long long med = s[s.size() / 2];
+  return (med <= 24) ? 1 : ((med <= 48) ? 2 : 4);
+}
+
+template <typename T, bool CPLX>
+int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vj_fint *info,
+             uint64_t *transforms, hipStream_t s)
+{
+  BatchArgs<T> a{};
+  const long long nb = P->nb;
+  a.nb = P->nb;
+  a.m = P->d_meta;
+  a.n = P->d_meta + nb;
+  a.offG = P->d_meta + 2 * nb;
+  a.offV = P->d_meta + 3 * nb;
+  a.offS = P->d_meta + 4 * nb;
+  a.tab_off = P->d_meta + 5 * nb;
+  a.order = P->d_order;
+  a.tables = P->d_tables;
+  a.Gr = (T *)Gr;
+  a.Gi = (T *)Gi;
+  a.Vr = (T *)Vr;
+  a.Vi = (T *)Vi;
+  a.eS = (T *)eS;
+  a.fS = (T *)fS;
+  a.info = (long long *)info;
+  a.transforms = (unsigned long long *)transforms;
+  a.queue = P->d_queue;
+  a.swp = P->swp;
+  // a fill kernel, not hipMemsetAsync, so that the run can be graph-captured
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, P->d_queue);
+  return batched_svd<T, CPLX>(a, P->waves, P->max_m, 0, s);
+}
+
+} // namespace
+
+extern "C" {
+
+void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG,
+                       const vj_fint *offV, const vj_fint *offS, unsigned swp, int waves, vj_fint *status)
+{
+  vj_fint dummy = 0;
+  vj_fint *st = status ? status : &dummy;
+  *st = 0;
+  if (kind < 0 || kind > 3 || nb < 0) {
+    *st = set_error(-1, "bad kind or batch size");
+    return nullptr;
+  }
+  std::vector<long long> hm(m, m + nb), hn(n, n + nb);
+  long long nmax = 0;
+  int max_m = 0;
+  for (vj_fint b = 0; b < nb; ++b) {
+    if (hn[b] < 2 || (hn[b] & 1) || hm[b] < hn[b] || hm[b] > 1024 || offG[b] < 0 || offV[b] < 0 || offS[b] < 0) {
+      *st = set_error(-2, "matrix %lld: need even n >= 2, n <= m <= 1024 and non-negative offsets (m=%lld n=%lld)",
+                      (long long)b, hm[b], hn[b]);
+      return nullptr;
+    }
+    nmax = std::max(nmax, hn[b]);
+    max_m = std::max(max_m, (int)hm[b]);
+  }
+  BPlan *P = new BPlan();
+  P->kind = kind;
+  P->nb = (int)nb;
+  P->max_m = max_m;
+  P->swp = swp;
+  P->waves = (waves > 0) ? waves : (nb ? pick_waves(hn) : 4);
+  (void)hipGetDevice(&P->dev);
+  // processing order: decreasing m n^2 (longest first)
+  std::vector<int> order((size_t)nb);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return hm[x] * hn[x] * hn[x] > hm[y] * hn[y] * hn[y];
+  });
+  // ME tables of every distinct width
+  std::vector<long long> tab_off((size_t)nmax + 1, 0);
+  std::vector<unsigned> tables;
+  std::map<long long, bool> widths;
+  for (long long w : hn)
+    widths[w] = true;
+  for (auto &kv : widths) {
+    unsigned stp = 0;
+    unsigned *t = pjs(2l, (unsigned)kv.first, &stp);
+    if (!t) {
+      delete P;
+      *st = set_error(-2, "no strategy table for n = %lld", kv.first);
+      return nullptr;
+    }
+    tab_off[(size_t)kv.first] = (long long)tables.size();
+    tables.insert(tables.end(), t, t + (size_t)stp * (size_t)kv.first);
+    free(t);
+  }
+  std::vector<long long> meta;
+  meta.reserve((size_t)(5 * nb) + tab_off.size());
+  meta.insert(meta.end(), hm.begin(), hm.end());
+  meta.insert(meta.end(), hn.begin(), hn.end());
+  meta.insert(meta.end(), offG, offG + nb);
+  meta.insert(meta.end(), offV, offV + nb);
+  meta.insert(meta.end(), offS, offS + nb);
+  meta.insert(meta.end(), tab_off.begin(), tab_off.end());
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc((void **)&P->d_meta, std::max<size_t>(1, meta.size()) * sizeof(long long));
+  if (e == hipSuccess) e = hipMalloc((void **)&P->d_order, std::max<size_t>(1, order.size()) * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void **)&P->d_tables, std::max<size_t>(1, tables.size()) * sizeof(unsigned));
+  if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, sizeof(unsigned));
+  if (e == hipSuccess && !meta.empty()) e = hipMemcpy(P->d_meta, meta.data(), meta.size() * sizeof(long long), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !order.empty()) e = hipMemcpy(P->d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !tables.empty()) e = hipMemcpy(P->d_tables, tables.data(), tables.size() * sizeof(unsigned), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    *st = set_error(VJ_EHIP - (int)e, "plan allocation failed: %s", hipGetErrorString(e));
+    (void)hipFree(P->d_meta);
+    (void)hipFree(P->d_order);
+    (void)hipFree(P->d_tables);
+    (void)hipFree(P->d_queue);
+    delete P;
+    return nullptr;
+  }
+  return P;
+}
+
+vj_fint vjh_bplan_run(void *plan, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vj_fint *info,
+                      uint64_t *transforms, void *stream)
+{
+  const BPlan *P = (const BPlan *)plan;
+  if (!P)
+    return set_error(-1, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  switch (P->kind) {
+  case 0: return run_plan<double, false>(P, Gr, nullptr, Vr, nullptr, eS, fS, info, transforms, s);
+  case 1: return run_plan<double, true>(P, Gr, Gi, Vr, Vi, eS, fS, info, transforms, s);
+  case 2: return run_plan<float, false>(P, Gr, nullptr, Vr, nullptr, eS, fS, info, transforms, s);
+  default: return run_plan<float, true>(P, Gr, Gi, Vr, Vi, eS, fS, info, transforms, s);
+  }
+}
+
+int vjh_bplan_waves(const void *plan) { return plan ? ((const BPlan *)plan)->waves : 0; }
+
+void vjh_bplan_destroy(void *plan)
+{
+  BPlan *P = (BPlan *)plan;
+  if (!P)
+    return;
+  (void)hipFree(P->d_meta);
+  (void)hipFree(P->d_order);
+  (void)hipFree(P->d_tables);
+  (void)hipFree(P->d_queue);
+  delete P;
+}
+
+vj_fint vjh_dbvjsvd(vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG, const vj_fint *offV,
+                    const vj_fint *offS, double *G, double *V, double *eS, double *fS, vj_fint *info,
+                    uint64_t *transforms, unsigned swp, int waves, void *stream)
+{
+  vj_fint st = 0;
+  void *P = vjh_bplan_create(0, nb, m, n, offG, offV, offS, swp, waves, &st);
+  if (!P)
+    return st;
+  const vj_fint rc = vjh_bplan_run(P, G, nullptr, V, nullptr, eS, fS, info, transforms, stream);
+  (void)hipStreamSynchronize((hipStream_t)stream);
+  vjh_bplan_destroy(P);
+  return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,310 @@
+// Batched one-sided Jacobi SVD of many small matrices of uneven sizes in one
+// persistent launch (no reference counterpart; each matrix follows
+// dvjsvd_/zvjsvd_, dvjsvd.c:40-638, with V := I).
+//
+// Workgroups pull matrices from a queue sorted by decreasing cost (m n^2),
+// so the long ones start first and the short ones fill the tail.  A
+// workgroup of W waves owns one matrix at a time and runs every sweep of it
+// without leaving the kernel: in each step the n/2 pairs of the strategy
+// table are dealt to the waves, each wave runs the whole pair step with
+// wave-level reductions only (pair_step with B = 64), and one workgroup
+// barrier separates consecutive steps.  The per-matrix control state
+// (max-|.| bounds, shifts, transformation counts) is kept in registers and
+// LDS; convergence is decided on the device.
+#include "svd_common.hpp"
+#include "svd_batched.hpp"
+
+namespace vj {
+
+template <typename T, bool CPLX, int E, int W>
+__global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
+{
+  constexpr int B = 64;
+  __shared__ int s_b;
+  __shared__ double s_max[2][W][2];            // per-wave (tG, tV) maxima, by step parity
+  __shared__ unsigned long long s_cnt[2][W];   // per-wave transformation counts, by step parity
+  __shared__ int s_err[2][W];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rot = CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R;
+
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0)
+      s_b = (int)atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const int bi = s_b;
+    if (bi >= a.nb)
+      break;
+    const int b = a.order[bi];
+    const int m = (int)a.m[b], n = (int)a.n[b];
+    T *const Gr = a.Gr + a.offG[b];
+    T *const Gi = CPLX ? a.Gi + a.offG[b] : nullptr;
+    T *const Vr = a.Vr + a.offV[b];
+    T *const Vi = CPLX ? a.Vi + a.offV[b] : nullptr;
+    T *const eS = a.eS + a.offS[b];
+    T *const fS = a.fS + a.offS[b];
+    const unsigned *const js = a.tables + a.tab_off[n];
+    const unsigned stp = (unsigned)(n - 1);
+
+    // ---- setup (dvjsvd.c:78-184): max |G|, V := I, eS/fS, initial scale
+    T mxv = (T)0;
+    for (long long t = threadIdx.x; t < (long long)m * n; t += 64 * W) {
+      mxv = xmax(xmin(vabs(Gr[t]), inf_<T>()), mxv);
+      if constexpr (CPLX)
+        mxv = xmax(xmin(vabs(Gi[t]), inf_<T>()), mxv);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+      mxv = xmax(mxv, __shfl_xor(mxv, o, 64));
+    if (lane == 0)
+      s_max[0][wave][0] = (double)mxv;
+    __syncthreads();
+    T MG = (T)s_max[0][0][0];
+    for (int w = 1; w < W; ++w)
+      MG = xmax(MG, (T)s_max[0][w][0]);
+    __syncthreads();
+    int status = 0;
+    if (!(MG <= Lim<T>::MAX))
+      status = CPLX ? -92 : -80;
+    if (!status) {
+      for (int t = threadIdx.x; t < n * n; t += 64 * W) {
+        const int j = t / n, i = t - j * n;
+        Vr[t] = (i == j) ? (T)1 : (T)0;
+        if constexpr (CPLX)
+          Vi[t] = (T)0;
+      }
+      for (int j = threadIdx.x; j < n; j += 64 * W) {
+        eS[j] = -inf_<T>();
+        fS[j] = (T)1;
+      }
+    }
+    unsigned sw = 0;
+    unsigned long long total = 0;
+    int sT = 0, NRM_EXP = 0;
+    T MV = (T)1;
+    if (!status && (MG > (T)0)) {
+      const T M_m = CPLX ? (Lim<T>::MAX / ((T)(m << 2) * (T)1.41421356237309504880)) : (Lim<T>::MAX / (T)(m << 1));
+      NRM_EXP = frexp_exp(M_m) - 1;
+      const int eM = frexp_exp(MG) - 1;
+      const int sN = NRM_EXP - eM - 1;
+      MG = ldexp_(MG, sN);
+      sT = sN;
+      if (sN) {
+        for (long long t = threadIdx.x; t < (long long)m * n; t += 64 * W) {
+          Gr[t] = ldexp_(Gr[t], sN);
+          if constexpr (CPLX)
+            Gi[t] = ldexp_(Gi[t], sN);
+        }
+      }
+      __syncthreads();
+      Step<T> S;
+      S.tol = sqrt_((T)m) * (Lim<T>::EPS / (T)2);
+      S.gst = ldexp_(S.tol, (int)Lim<T>::FIN_EXP);
+      S.sG = 0;
+      S.sV = 0;
+      S.vtrack = 0;
+      int par = 0;
+      for (sw = 0; sw < a.swp;) {
+        unsigned long long swt = 0;
+        for (unsigned st = 0; st < stp; ++st) {
+          const unsigned *const row = js + (size_t)st * n;
+          T wG = (T)0, wV = (T)0;
+          unsigned long long wcnt = 0;
+          int werr = 0;
+          for (int k = wave; k < (n >> 1); k += W) {
+            const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
+            T *const Gp = Gr + (long long)p * m, *const Gq = Gr + (long long)q * m;
+            T *const Gpi = CPLX ? Gi + (long long)p * m : nullptr, *const Gqi = CPLX ? Gi + (long long)q * m : nullptr;
+            T xr[E], yr[E], xi[E], yi[E];
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+              const int i = lane + j * B;
+              const bool in = (i < m);
+              xr[j] = in ? Gp[i] : (T)0;
+              yr[j] = in ? Gq[i] : (T)0;
+              if constexpr (CPLX) {
+                xi[j] = in ? Gpi[i] : (T)0;
+                yi[j] = in ? Gqi[i] : (T)0;
+              }
+              else
+                xi[j] = yi[j] = (T)0;
+              if (S.sG) {
+                xr[j] = ldexp_(xr[j], S.sG);
+                yr[j] = ldexp_(yr[j], S.sG);
+                if constexpr (CPLX) {
+                  xi[j] = ldexp_(xi[j], S.sG);
+                  yi[j] = ldexp_(yi[j], S.sG);
+                }
+              }
+            }
+            T *const Vp = Vr + (long long)p * n, *const Vq = Vr + (long long)q * n;
+            T *const Vpi = CPLX ? Vi + (long long)p * n : nullptr, *const Vqi = CPLX ? Vi + (long long)q * n : nullptr;
+            if (S.sV) {
+              for (int i = lane; i < n; i += B) {
+                Vp[i] = ldexp_(Vp[i], S.sV);
+                Vq[i] = ldexp_(Vq[i], S.sV);
+                if constexpr (CPLX) {
+                  Vpi[i] = ldexp_(Vpi[i], S.sV);
+                  Vqi[i] = ldexp_(Vqi[i], S.sV);
+                }
+              }
+            }
+            T eSp, fSp, eSq, fSq;
+            const PairOut<T, CPLX> o =
+                pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, nullptr, lane);
+            if (o.err) {
+              werr = o.err;
+              continue;
+            }
+            if ((o.code != 1) || S.sG) {
+#pragma unroll
+              for (int j = 0; j < E; ++j) {
+                const int i = lane + j * B;
+                if (i < m) {
+                  Gp[i] = xr[j];
+                  Gq[i] = yr[j];
+                  if constexpr (CPLX) {
+                    Gpi[i] = xi[j];
+                    Gqi[i] = yi[j];
+                  }
+                }
+              }
+            }
+            if (lane == 0) {
+              eS[p] = eSp; fS[p] = fSp;
+              eS[q] = eSq; fS[q] = fSq;
+            }
+            wG = xmax(wG, o.tG);
+            wV = xmax(wV, o.tV);
+            wcnt += (unsigned long long)o.trans;
+          }
+          if (lane == 0) {
+            s_max[par][wave][0] = (double)wG;
+            s_max[par][wave][1] = (double)wV;
+            s_cnt[par][wave] = wcnt;
+            s_err[par][wave] = werr;
+          }
+          __syncthreads();
+          // every thread folds the same LDS values: uniform decisions
+          T nMG = (T)0, nMV = (T)0;
+          unsigned long long cnt = 0;
+          int err = 0;
+          for (int w = 0; w < W; ++w) {
+            nMG = xmax(nMG, (T)s_max[par][w][0]);
+            nMV = xmax(nMV, (T)s_max[par][w][1]);
+            cnt += s_cnt[par][w];
+            err = err ? err : s_err[par][w];
+          }
+          par ^= 1;
+          if (err) {
+            status = err;
+            break;
+          }
+          swt += cnt;
+          MG = (nMG > MG) ? nMG : MG;
+          MV = (nMV > MV) ? nMV : MV;
+          S.sG = 0;
+          S.sV = 0;
+          {
+            const int eM = frexp_exp(MG) - 1;
+            if (rot - eM < 0) {
+              S.sG = NRM_EXP - eM - 1;
+              MG = ldexp_(MG, S.sG);
+              sT += S.sG;
+            }
+            const int eV = frexp_exp(MV) - 1;
+            if (rot - eV < 0) {
+              S.sV = rot - eV;
+              MV = ldexp_(MV, S.sV);
+            }
+          }
+          __threadfence_block();
+        }
+        if (status)
+          break;
+        total += swt;
+        if (!swt)
+          break;
+        ++sw;
+      }
+      __syncthreads();
+      // ---- epilogue (dvjsvd.c:604-631); V is not rescaled when it starts as I
+      if (!status && (sw < a.swp)) {
+        for (long long t = threadIdx.x; t < (long long)m * n; t += 64 * W) {
+          const int j = (int)(t / m);
+          const T f_ = fS[j], s_ = -eS[j];
+          Gr[t] = scalef(Gr[t] / f_, s_);
+          if constexpr (CPLX)
+            Gi[t] = scalef(Gi[t] / f_, s_);
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < n; j += 64 * W)
+          eS[j] -= (T)sT;
+      }
+    }
+    if (threadIdx.x == 0) {
+      a.info[b] = status ? (long long)status : (long long)sw;
+      if (a.transforms)
+        a.transforms[b] = total;
+    }
+  }
+}
+
+template <typename T, bool CPLX, int E, int W>
+static int launch_bsvd(const BatchArgs<T> &a, int grid, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_bsvd<T, CPLX, E, W>), dim3(grid), dim3(64 * W), 0, s, a);
+  return check_launch();
+}
+
+template <typename T, bool CPLX, int E, int W>
+static int occupancy(int *blocks)
+{
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_bsvd<T, CPLX, E, W>, 64 * W, 0);
+  return (e == hipSuccess) ? 0 : set_error(VJ_EHIP - (int)e, "occupancy query failed");
+}
+
+template <typename T, bool CPLX, int W>
+static int dispatch_E(const BatchArgs<T> &a, int max_m, int grid_cap, hipStream_t s)
+{
+  int cus = 0, blocks = 1, dev = 0;
+  VJ_HIP(hipGetDevice(&dev));
+  VJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+#define VJ_BS_CASE(EE)                                                         \
+  {                                                                            \
+    if (int rc = occupancy<T, CPLX, EE, W>(&blocks))                           \
+      return rc;                                                               \
+    long long g = (long long)cus * (blocks > 0 ? blocks : 1);                  \
+    if (grid_cap > 0 && g > grid_cap) g = grid_cap;                            \
+    if (g > a.nb) g = a.nb;                                                    \
+    return launch_bsvd<T, CPLX, EE, W>(a, (int)g, s);                          \
+  }
+  if (max_m <= 64) VJ_BS_CASE(1)
+  if (max_m <= 128) VJ_BS_CASE(2)
+  if (max_m <= 256) VJ_BS_CASE(4)
+  if (max_m <= 512) VJ_BS_CASE(8)
+  if (max_m <= 1024) VJ_BS_CASE(16)
+#undef VJ_BS_CASE
+  return set_error(-1, "batched SVD: m = %d exceeds 1024", max_m);
+}
+
+template <typename T, bool CPLX>
+int batched_svd(const BatchArgs<T> &a, int waves, int max_m, int grid_cap, hipStream_t s)
+{
+  if (a.nb <= 0)
+    return 0;
+  switch (waves) {
+  case 1: return dispatch_E<T, CPLX, 1>(a, max_m, grid_cap, s);
+  case 2: return dispatch_E<T, CPLX, 2>(a, max_m, grid_cap, s);
+  case 4: return dispatch_E<T, CPLX, 4>(a, max_m, grid_cap, s);
+  case 8: return dispatch_E<T, CPLX, 8>(a, max_m, grid_cap, s);
+  default: return set_error(-1, "batched SVD: waves per workgroup must be 1, 2, 4 or 8 (got %d)", waves);
+  }
+}
+
+template int batched_svd<double, false>(const BatchArgs<double> &, int, int, int, hipStream_t);
+template int batched_svd<double, true>(const BatchArgs<double> &, int, int, int, hipStream_t);
+template int batched_svd<float, false>(const BatchArgs<float> &, int, int, int, hipStream_t);
+template int batched_svd<float, true>(const BatchArgs<float> &, int, int, int, hipStream_t);
+
+} // namespace vj

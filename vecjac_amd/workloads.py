@@ -69,3 +69,30 @@ def batch_shapes(nb: int, seed: int, nmin: int = 8, nmax: int = 128):
     n = g.integers(nmin, nmax + 1, nb)
     m = np.array([g.integers(int(k), 4 * int(k) + 1) for k in n], dtype=np.int64)
     return m.astype(np.int64), n.astype(np.int64)
+
+
+def gapped_sigma(n: int, g: np.random.Generator, lo: float = 1.0, hi: float = 10.0, gap: float = 1e-3):
+    """n values ~ U[lo, hi] whose sorted neighbours differ by at least
+    gap * (larger value) (rejection sampling)."""
+    while True:
+        s = np.sort(g.uniform(lo, hi, n))[::-1]
+        if n < 2 or np.all((s[:-1] - s[1:]) >= gap * s[:-1]):
+            return s
+
+
+def grad_batch(nb: int, m: int, n: int, seed: int):
+    """Differentiable-SVD workload: A_b = Q1 diag(sigma) Q2^T with Haar Q1
+    (m x n), Q2 (n x n), sigma ~ U[1, 10] with minimum relative gap 1e-3;
+    loss weights w (n), G_U (m x n), G_V (n x n) seeded Gaussian.
+    Returns A (nb, m, n), w, GU, GV as float64 arrays."""
+    g = rng(seed)
+    A = np.empty((nb, m, n))
+    for b in range(nb):
+        s = gapped_sigma(n, g)
+        Q1 = haar_unitary(m, n, g, False)
+        Q2 = haar_unitary(n, n, g, False)
+        A[b] = (Q1 * s) @ Q2.T
+    w = g.standard_normal((nb, n))
+    GU = g.standard_normal((nb, m, n))
+    GV = g.standard_normal((nb, n, n))
+    return A, w, GU, GV
