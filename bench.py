@@ -144,9 +144,9 @@ def bench_bsvd(args, kind: str = "d"):
     from vecjac_amd import workloads as W
 
     nb = args.nb
-    m, n_real = W.batch_shapes(nb, 4242, 8, 128)
-    m_, npad, offG, offV, offS, tot = BT.pack_shapes(m, n_real)
-    G64 = BT.fill_gaussian(m, npad, n_real, offG, 4243, np.float64)
+    m_real, n_real = W.batch_shapes(nb, 4242, 8, 128)
+    m, npad, offG, offV, offS, tot = BT.pack_shapes(m_real, n_real)
+    G64 = BT.fill_gaussian(m, npad, n_real, offG, 4243, np.float64, m_real=m_real)
 
     def run_batch(kind, steps, warmup, timed=True):
         dt_np = np.float64 if kind == "d" else np.float32
@@ -216,7 +216,7 @@ def bench_bsvd(args, kind: str = "d"):
         r32["plan"].close()
     r["plan"].close()
     if not args.no_cpu:
-        out["cpu_baseline"] = _bsvd_cpu(m, n_real, offG, npad, G64, kind)
+        out["cpu_baseline"] = _bsvd_cpu(m_real, n_real, m, offG, npad, G64, kind)
     return out
 
 
@@ -269,7 +269,7 @@ def _bsvd_plain_sample(m, npad, offG, G64, kind, nsamp):
             "vjh_dvjsvd (one launch per step)"}
 
 
-def _bsvd_cpu(m, n_real, offG, npad, G64, kind):
+def _bsvd_cpu(m_real, n_real, m, offG, npad, G64, kind):
     """Reference dvjsvd_ (its own sources, AVX-512; long-double norm) over a
     thread pool, one matrix per call, inputs zero-padded to m%8 == 0 and
     n%16 == 0 as the reference requires; the oracle port when the reference
@@ -281,7 +281,7 @@ def _bsvd_cpu(m, n_real, offG, npad, G64, kind):
     threads = _cpu_threads()
     use_ref = vjo.ref() is not None and _host_has_avx512() and kind == "d"
     g = np.random.Generator(np.random.Philox(9))
-    order = g.permutation(len(m))
+    order = g.permutation(len(m_real))
     budget = 15.0
     if use_ref:
         R = vjo.ref()
@@ -291,12 +291,12 @@ def _bsvd_cpu(m, n_real, offG, npad, G64, kind):
             gomp.omp_set_num_threads(1)
 
         def one(b):
-            mm, nn = int(m[b]), int(n_real[b])
-            m8 = (mm + 7) // 8 * 8
+            mm, nn, mr = int(m[b]), int(n_real[b]), int(m_real[b])
+            m8 = (mr + 7) // 8 * 8
             n16 = max(16, (nn + 15) // 16 * 16)
             m8 = max(m8, n16)
             A = vjo.aligned((n16, m8), np.float64)
-            A[:nn, :mm] = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
+            A[:nn, :mr] = G64[offG[b]: offG[b] + mm * int(npad[b])].reshape(int(npad[b]), mm)[:nn, :mr]
             js, stp = S.pjs(2, n16)
             jsa = vjo.aligned(js.shape, np.uint32)
             jsa[...] = js
@@ -334,7 +334,95 @@ def _bsvd_cpu(m, n_real, offG, npad, G64, kind):
                          if use_ref else "oracle C restatement, one matrix per thread") + ")"}
 
 
+def bench_plain(args, kind: str):
+    """Plain SVD of one large matrix through the device driver (one launch
+    per step): cfg1 (dvjsvd, 1024 x 1024 N(0,1)) or cfg3 (zvjsvd, 8192 x 2048,
+    A = Q1 diag(sigma) Q2^H, sigma geometric from 1 to 1e-12)."""
+    import torch
+    from vecjac_amd import svd as S
+    from vecjac_amd import workloads as W
+    if kind == "d":
+        m, n = args.m or 1024, args.n or 1024
+        A = W.gaussian_matrix(m, n, 1024)
+        Ar, Ai = A, None
+        desc = f"plain dvjsvd, FP64 {m}x{n}, N(0,1), ME table, swp=999, V=I"
+    else:
+        m, n = args.m or 8192, args.n or 2048
+        sig = 10.0 ** (-12.0 * np.arange(n) / (n - 1))
+        A = W.graded_svd_matrix(m, n, sig, 8192, True)  # m x n complex
+        Ar = np.ascontiguousarray(A.real.T)
+        Ai = np.ascontiguousarray(A.imag.T)
+        desc = f"plain zvjsvd, complex FP64 {m}x{n}, A = Q1 diag(10^(-12 i/(n-1))) Q2^H, Haar Q1 Q2, ME table"
+    js, stp = S.pjs(2, n)
+    jsd = torch.from_numpy(js.reshape(-1).view(np.int32)).cuda()
+    G0 = torch.from_numpy(np.ascontiguousarray(Ar)).cuda()
+    Gi0 = torch.from_numpy(Ai).cuda() if Ai is not None else None
+    G = torch.empty_like(G0)
+    Gi = torch.empty_like(Gi0) if Gi0 is not None else None
+    V = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    Vi = torch.empty((n, n), dtype=torch.float64, device="cuda") if Ai is not None else None
+    eS = torch.empty(n, dtype=torch.float64, device="cuda")
+    fS = torch.empty(n, dtype=torch.float64, device="cuda")
+    walls, sweeps, hist = [], None, None
+    for it in range(args.warmup + args.steps):
+        G.copy_(G0)
+        if Gi is not None:
+            Gi.copy_(Gi0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc, st = S.device_vjsvd(kind, G, V, eS, fS, jsd, stp, 999, Gi=Gi, Vi=Vi)
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            walls.append(time.perf_counter() - t0)
+        sweeps, hist = rc, st
+    wall = float(np.mean(walls))
+    w = 8 if kind == "d" else 16
+    R = float(np.sum(hist[2][: max(0, sweeps) + 1]))
+    steps_run = (sweeps + 1) * stp
+    alg = steps_run * w * m * n + R * w * (2 * m + 4 * n)
+    achieved = alg / wall / 1e9
+    return {"metric": "SVDs/s (plain, one matrix)", "value": 1.0 / wall, "unit": "SVDs/s",
+            "ms_per_step": wall * 1e3, "dtype": "f64" if kind == "d" else "c128",
+            "config": {"workload": desc, "sweeps": int(sweeps), "launched_sweeps": int(hist[1]),
+                       "transformations": R},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step (all launches)",
+                         "kernel_avg_ms": wall / max(1, int(hist[1]) * stp) * 1e3, "alg_bytes_per_launch": alg / max(1, steps_run)}}
+
+
+def bench_copy(args):
+    """Plain device-to-device copy of 2^lg doubles: the calibration kernel
+    for the FETCH_SIZE / WRITE_SIZE counters (its traffic is known)."""
+    import torch
+    n = 1 << int(args.lg)
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        b.copy_(a)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    k = ev[0].elapsed_time(ev[1]) / args.steps / 1e3
+    bytes_ = 2 * 8 * n
+    return {"metric": "device copy bandwidth (calibration)", "value": bytes_ / k / 1e9, "unit": "GB/s",
+            "ms_per_step": wall / args.steps * 1e3, "dtype": "f64",
+            "config": {"workload": f"copy of 2^{args.lg} doubles", "bytes_per_copy": bytes_},
+            "roofline": {"bound": "hbm", "achieved": bytes_ / k / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": bytes_ / k / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": "copy",
+                         "kernel_avg_ms": k * 1e3, "alg_bytes_per_launch": bytes_}}
+
+
 WORKLOADS = {
+    "copy": bench_copy,
+    "svd1024": lambda a: bench_plain(a, "d"),
+    "zsvd": lambda a: bench_plain(a, "z"),
     "bsvd": lambda a: bench_bsvd(a, "d"),
     "bsvd32": lambda a: bench_bsvd(a, "s"),
     "bjac2": lambda a: bench_bjac2(a, "d"),
@@ -356,6 +444,8 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--m", type=int, default=0, help="rows of the plain workloads (0: the config's)")
+    ap.add_argument("--n", type=int, default=0, help="columns of the plain workloads (0: the config's)")
     ap.add_argument("--lg", type=int, default=26, help="log2 batch size of the 2x2 workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()

@@ -96,9 +96,76 @@ T CN(norm2)(const vjo_fnat m, const T *const xr, const T *const xi, T *const e0,
   return (T)sq;
 }
 
+/* The GPU's scaled dot product, restated for comparison runs (mode 1):
+ * elements scaled by 2^-k with k the exponent of each column's max |.|,
+ * products accumulated exactly enough (double-double), the sum rounded once
+ * and rescaled by the exact power of two that turns the max-exponent
+ * scaling into the reference's norm-exponent scaling (ddpscl.c:36-37), then
+ * divided by fx fy as the reference does (ddpscl.c:73-74).  Complex:
+ * x^H y.  The result is the reference's quantity correctly rounded (up to
+ * ties), independent of the summation order. */
+static void SFX(dot_cr)(const vjo_fnat m, const T *const xr, const T *const xi, const T *const yr, const T *const yi,
+                        const T *const e, const T *const f, T *const zr, T *const zi)
+{
+  *zr = (T)0;
+  if (zi)
+    *zi = (T)0;
+  if (!(e[0u] >= -(T)TMAX) || !(e[1u] >= -(T)TMAX))
+    return;
+  T mx = (T)0, my = (T)0;
+  for (vjo_fnat i = 0u; i < m; ++i) {
+    mx = SFX(xmax)(mx, SFX(abs)(xr[i]));
+    my = SFX(xmax)(my, SFX(abs)(yr[i]));
+    if (xi) {
+      mx = SFX(xmax)(mx, SFX(abs)(xi[i]));
+      my = SFX(xmax)(my, SFX(abs)(yi[i]));
+    }
+  }
+  const int kx = ilogb((double)mx), ky = ilogb((double)my);
+  double sr = 0.0, cr = 0.0, si = 0.0, ci = 0.0;
+#define VJO_ACC(S, C, P)                        \
+  {                                             \
+    const double p_ = (P);                      \
+    double t_;                                  \
+    two_sum_d(S, p_, &S, &t_);                  \
+    C += t_;                                    \
+  }
+  for (vjo_fnat i = 0u; i < m; ++i) {
+    const double a = ldexp((double)xr[i], -kx), b = ldexp((double)yr[i], -ky);
+    /* exact products as a + e with e = fma(.,.,-p); for FP32 data the
+     * product is exact in double already */
+    double p = a * b;
+    VJO_ACC(sr, cr, p);
+    cr += fma(a, b, -p);
+    if (xi) {
+      const double c = ldexp((double)xi[i], -kx), d = ldexp((double)yi[i], -ky);
+      p = c * d;
+      VJO_ACC(sr, cr, p);
+      cr += fma(c, d, -p);
+      p = a * d;
+      VJO_ACC(si, ci, p);
+      ci += fma(a, d, -p);
+      p = -(c * b);
+      VJO_ACC(si, ci, p);
+      ci -= fma(c, b, -(c * b));
+    }
+  }
+#undef VJO_ACC
+  const int shift = (kx + ky) - (int)(e[0u] + e[1u]);
+  const T de = (f[0u] * f[1u]);
+  *zr = (T)ldexp((double)(T)(sr + cr), shift) / de;
+  if (zi)
+    *zi = (T)ldexp((double)(T)(si + ci), shift) / de;
+}
+
 /* ddpscl_ (ddpscl.c:7-75) with USE_2SUM */
 T RN(dpscl)(const vjo_fnat m, const T *const x, const T *const y, const T *const e, const T *const f)
 {
+  if (vjo_norm_mode) {
+    T z = (T)0;
+    SFX(dot_cr)(m, x, (const T*)NULL, y, (const T*)NULL, e, f, &z, (T*)NULL);
+    return z;
+  }
   const T ex = e[0u];
   if (!(ex >= -(T)TMAX))
     return (T)0;
@@ -130,6 +197,10 @@ T RN(dpscl)(const vjo_fnat m, const T *const x, const T *const y, const T *const
 void CN(dpscl)(const vjo_fnat m, const T *const xr, const T *const xi, const T *const yr, const T *const yi, const T *const e, const T *const f, T *const zr, T *const zi)
 {
   *zr = *zi = (T)0;
+  if (vjo_norm_mode) {
+    SFX(dot_cr)(m, xr, xi, yr, yi, e, f, zr, zi);
+    return;
+  }
   const T ex = e[0u];
   if (!(ex >= -(T)TMAX))
     return;

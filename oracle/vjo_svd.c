@@ -19,8 +19,9 @@ static inline int vjo_fpenv_bad(void)
 }
 #define VJO_MISALIGNED(x) (((uintptr_t)(x)) & 0x3Fu)
 
-/* 0: column norms as the reference's long-double dnormf_ (bit-identical to
- * the reference build); 1: the GPU's correctly rounded norm */
+/* 0: column norms as the reference's long-double dnormf_ and dots as its
+ * ddpscl_/zdpscl_ (bit-identical to the reference build); 1: the GPU's
+ * correctly rounded norms and dots */
 static int vjo_norm_mode = 0;
 void vjo_set_norm_mode(int mode) { vjo_norm_mode = mode; }
 int vjo_get_norm_mode(void) { return vjo_norm_mode; }

@@ -34,11 +34,13 @@ def _declare(L):
 
 
 def pack_shapes(m, n):
-    """Offsets for packed storage; odd widths are padded to even.
-    Returns (m, n_padded, offG, offV, offS, totals)."""
+    """Offsets for packed storage; odd widths are padded to even (and a
+    square odd matrix gets a zero row as well, so that n <= m still holds).
+    Returns (m_padded, n_padded, offG, offV, offS, totals)."""
     m = np.asarray(m, dtype=np.int64)
     n = np.asarray(n, dtype=np.int64)
     npad = n + (n & 1)
+    m = np.maximum(m, npad)
     sizeG = m * npad
     sizeV = npad * npad
     offG = np.concatenate([[0], np.cumsum(sizeG)[:-1]]).astype(np.int64)
@@ -87,16 +89,20 @@ class BatchedSVD:
             pass
 
 
-def fill_gaussian(m, n, n_real, offG, seed: int, dtype=np.float64):
-    """Packed G with N(0,1) entries for the first n_real[b] columns of each
-    matrix and zeros in its padding column (if any)."""
+def fill_gaussian(m, n, n_real, offG, seed: int, dtype=np.float64, m_real=None):
+    """Packed G (padded shapes m x n) with N(0,1) entries in the leading
+    m_real[b] x n_real[b] block of each matrix and zeros in its padding."""
     g = np.random.Generator(np.random.Philox(seed))
     m = np.asarray(m)
     n = np.asarray(n)
+    m_real = m if m_real is None else np.asarray(m_real)
     total = int(np.sum(m * n))
     G = g.standard_normal(total)  # float64 draws, so the FP32 batch is the same batch rounded
     if G.dtype != dtype:
         G = G.astype(dtype)
-    for b in np.nonzero(np.asarray(n_real) != n)[0]:
-        G[offG[b] + int(m[b]) * int(n_real[b]): offG[b] + int(m[b]) * int(n[b])] = 0
+    for b in np.nonzero((np.asarray(n_real) != n) | (m_real != m))[0]:
+        mm, nn = int(m[b]), int(n[b])
+        blk = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
+        blk[int(n_real[b]):, :] = 0
+        blk[:, int(m_real[b]):] = 0
     return G

@@ -66,8 +66,9 @@ __global__ __launch_bounds__(256) void k_svd_bwd(const int nb, const int m, cons
       else {
         const T si = ss[i], sj = ss[j];
         const T F = (T)1 / ((sj - si) * (sj + si));
-        const T aJ = sJ[t] - sJ[j * n + i];
-        const T aK = sN[t] - sN[j * n + i];
+        // column-major: (i, j) at t = i + j n, (j, i) at j + i n
+        const T aJ = sJ[t] - sJ[i * n + j];
+        const T aK = sN[t] - sN[i * n + j];
         v = F * (aJ * sj + si * aK) - sJ[t] / sj;
       }
       nv[cnt++] = v;
