@@ -8,7 +8,10 @@ from __future__ import annotations
 import ctypes as C
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().with_name("libvecjac_hip.so")
+import os
+
+# VJ_LIB selects another build of the same library (for comparing builds)
+LIB_PATH = Path(os.environ.get("VJ_LIB") or Path(__file__).resolve().with_name("libvecjac_hip.so"))
 
 _fint = C.c_int64
 _fnat = C.c_uint64

@@ -4,7 +4,7 @@
 // be captured in a HIP graph.
 #include "svd_batched.hpp"
 #include "svd_common.hpp"
-#include "../../include/vecjac.h"
+#include "vecjac.h"
 
 #include <algorithm>
 #include <map>

@@ -7,7 +7,7 @@
 // FP32) because p[] packs that many L1<L2 bits per word; otherwise -1 (the
 // reference's own argument check, dbjac2.c:86-87).  Returns 0 on success.
 #include "capi_common.hpp"
-#include "../../include/vecjac.h"
+#include "vecjac.h"
 
 using namespace vj;
 

@@ -4,7 +4,7 @@
 //     reference's argument checks in the reference's order;
 //   vjh_?vjsvd: the same driver on device arrays and a device step table.
 #include "svd_common.hpp"
-#include "../../include/vecjac.h"
+#include "vecjac.h"
 
 #include <stdint.h>
 #include <string.h>

@@ -5,21 +5,79 @@
 // Workgroups pull matrices from a queue sorted by decreasing cost (m n^2),
 // so the long ones start first and the short ones fill the tail.  A
 // workgroup of W waves owns one matrix at a time and runs every sweep of it
-// without leaving the kernel: in each step the n/2 pairs of the strategy
-// table are dealt to the waves, each wave runs the whole pair step with
-// wave-level reductions only (pair_step with B = 64), and one workgroup
-// barrier separates consecutive steps.  The per-matrix control state
-// (max-|.| bounds, shifts, transformation counts) is kept in registers and
-// LDS; convergence is decided on the device.
+// without leaving the kernel.  Each step of the strategy table runs in three
+// phases separated by workgroup barriers:
+//   A  the waves share out the n/2 pairs; a wave loads its pair, computes the
+//      two norms and the dot product with wave-level reductions and leaves
+//      the (e, f) norms and the off-diagonal element in LDS;
+//   B  one lane per pair takes the rotate/swap/skip decision and solves the
+//      2x2 EVD, so the divisions and square roots run on all lanes at once
+//      instead of once per wave;
+//   C  the waves reload their pairs, apply the transformations to G and V
+//      and write the new column norms.
+// The per-matrix control state (max-|.| bounds, shifts, transformation
+// counts) is folded through double-buffered LDS slots, so every thread takes
+// the same rescaling and convergence decisions; no host round trip.
 #include "svd_common.hpp"
 #include "svd_batched.hpp"
 
 namespace vj {
 
+// Column pair (p, q) of an m-row matrix into registers, row lane + 64 j,
+// scaled by 2^sG when a rescaling is pending.
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ void load_pair(const T *Gr, const T *Gi, const int m, const int p, const int q, const int sG,
+                                          const int lane, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E])
+{
+  const T *const Gp = Gr + (long long)p * m, *const Gq = Gr + (long long)q * m;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int i = lane + j * 64;
+    const bool in = (i < m);
+    xr[j] = in ? Gp[i] : (T)0;
+    yr[j] = in ? Gq[i] : (T)0;
+    if constexpr (CPLX) {
+      xi[j] = in ? Gi[(long long)p * m + i] : (T)0;
+      yi[j] = in ? Gi[(long long)q * m + i] : (T)0;
+    }
+    else
+      xi[j] = yi[j] = (T)0;
+    if (sG) {
+      xr[j] = ldexp_(xr[j], sG);
+      yr[j] = ldexp_(yr[j], sG);
+      if constexpr (CPLX) {
+        xi[j] = ldexp_(xi[j], sG);
+        yi[j] = ldexp_(yi[j], sG);
+      }
+    }
+  }
+}
+
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ void store_pair(T *Gr, T *Gi, const int m, const int p, const int q, const int lane,
+                                           const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E])
+{
+  T *const Gp = Gr + (long long)p * m, *const Gq = Gr + (long long)q * m;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int i = lane + j * 64;
+    if (i < m) {
+      Gp[i] = xr[j];
+      Gq[i] = yr[j];
+      if constexpr (CPLX) {
+        Gi[(long long)p * m + i] = xi[j];
+        Gi[(long long)q * m + i] = yi[j];
+      }
+    }
+  }
+}
+
 template <typename T, bool CPLX, int E, int W>
 __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
 {
   constexpr int B = 64;
+  __shared__ PairScal<T> s_scal[64];           // phase A -> B/C (n <= 128)
+  __shared__ PairDec<T> s_dec[64];             // phase B -> C
   __shared__ int s_b;
   __shared__ double s_max[2][W][2];            // per-wave (tG, tV) maxima, by step parity
   __shared__ unsigned long long s_cnt[2][W];   // per-wave transformation counts, by step parity
@@ -108,35 +166,29 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
         unsigned long long swt = 0;
         for (unsigned st = 0; st < stp; ++st) {
           const unsigned *const row = js + (size_t)st * n;
+          const int npairs = n >> 1;
+          // ---- phase A: norms and cosines of this wave's pairs -> LDS
+          for (int k = wave; k < npairs; k += W) {
+            const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
+            T xr[E], yr[E], xi[E], yi[E];
+            load_pair<T, CPLX, E>(Gr, Gi, m, p, q, S.sG, lane, xr, yr, xi, yi);
+            const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, nullptr);
+            if (lane == 0)
+              s_scal[k] = P;
+          }
+          __syncthreads();
+          // ---- phase B: decisions and 2x2 EVDs of all pairs, one lane per pair
+          for (int k = threadIdx.x; k < npairs; k += 64 * W)
+            s_dec[k] = pair_decide<T, CPLX>(s_scal[k], S.tol, S.gst);
+          __syncthreads();
+          // ---- phase C: updates
           T wG = (T)0, wV = (T)0;
           unsigned long long wcnt = 0;
           int werr = 0;
-          for (int k = wave; k < (n >> 1); k += W) {
+          for (int k = wave; k < npairs; k += W) {
             const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
-            T *const Gp = Gr + (long long)p * m, *const Gq = Gr + (long long)q * m;
-            T *const Gpi = CPLX ? Gi + (long long)p * m : nullptr, *const Gqi = CPLX ? Gi + (long long)q * m : nullptr;
-            T xr[E], yr[E], xi[E], yi[E];
-#pragma unroll
-            for (int j = 0; j < E; ++j) {
-              const int i = lane + j * B;
-              const bool in = (i < m);
-              xr[j] = in ? Gp[i] : (T)0;
-              yr[j] = in ? Gq[i] : (T)0;
-              if constexpr (CPLX) {
-                xi[j] = in ? Gpi[i] : (T)0;
-                yi[j] = in ? Gqi[i] : (T)0;
-              }
-              else
-                xi[j] = yi[j] = (T)0;
-              if (S.sG) {
-                xr[j] = ldexp_(xr[j], S.sG);
-                yr[j] = ldexp_(yr[j], S.sG);
-                if constexpr (CPLX) {
-                  xi[j] = ldexp_(xi[j], S.sG);
-                  yi[j] = ldexp_(yi[j], S.sG);
-                }
-              }
-            }
+            const PairScal<T> P = s_scal[k];
+            const PairDec<T> D = s_dec[k];
             T *const Vp = Vr + (long long)p * n, *const Vq = Vr + (long long)q * n;
             T *const Vpi = CPLX ? Vi + (long long)p * n : nullptr, *const Vqi = CPLX ? Vi + (long long)q * n : nullptr;
             if (S.sV) {
@@ -150,25 +202,21 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
               }
             }
             T eSp, fSp, eSq, fSq;
-            const PairOut<T, CPLX> o =
-                pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, nullptr, lane);
+            PairOut<T, CPLX> o;
+            if ((D.code != 1) || S.sG || D.err) {
+              T xr[E], yr[E], xi[E], yi[E];
+              load_pair<T, CPLX, E>(Gr, Gi, m, p, q, S.sG, lane, xr, yr, xi, yi);
+              o = pair_apply<T, CPLX, B, E>(n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane);
+              if (!o.err)
+                store_pair<T, CPLX, E>(Gr, Gi, m, p, q, lane, xr, yr, xi, yi);
+            }
+            else {
+              o = PairOut<T, CPLX>{1, 0, 0, (T)0, (T)0};
+              eSp = P.ex; fSp = P.fx; eSq = P.ey; fSq = P.fy;
+            }
             if (o.err) {
               werr = o.err;
               continue;
-            }
-            if ((o.code != 1) || S.sG) {
-#pragma unroll
-              for (int j = 0; j < E; ++j) {
-                const int i = lane + j * B;
-                if (i < m) {
-                  Gp[i] = xr[j];
-                  Gq[i] = yr[j];
-                  if constexpr (CPLX) {
-                    Gpi[i] = xi[j];
-                    Gqi[i] = yi[j];
-                  }
-                }
-              }
             }
             if (lane == 0) {
               eS[p] = eSp; fS[p] = fSp;

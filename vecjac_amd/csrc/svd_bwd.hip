@@ -9,7 +9,7 @@
 // matrix; U, dU, V and the n x n intermediates are staged in LDS (64 KiB
 // per workgroup at most).
 #include "svd_common.hpp"
-#include "../../include/vecjac.h"
+#include "vecjac.h"
 
 namespace vj {
 

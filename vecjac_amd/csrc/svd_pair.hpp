@@ -1,22 +1,27 @@
-// One Jacobi step on one column pair, executed by a group of B lanes (one
-// wave for B = 64, a whole workgroup for B = 256/512/1024).
+// One Jacobi step on one column pair, in three phases, executed by a group of
+// B lanes (one wave for B = 64, a whole workgroup for B = 256/1024):
 //
-// The pair's columns are held in registers (E elements per lane, row
-// lane + j*B), so a step reads each column once and writes it once:
-//   1. exact column norms in (e,f) form: max-|.| exponent scaling, then a
-//      double-double sum of squares and a corrected square root
-//      (replaces dnorm2_/znorm2_, dnorm2.c:9-14, znorm2.c:9-15);
-//   2. the scaled, compensated dot product (replaces ddpscl_/zdpscl_,
-//      ddpscl.c:7-75, zdpscl.c:21-116): TwoProd + TwoSum per element and a
-//      double-double tree reduction;
-//   3. the convergence test, Gram prescaling and action code
-//      (dvjsvd.c:349-459) and the 2x2 EVD (dbjac2i cosine form);
-//   4. the in-place update: rotation (djrot_/djrotf_/zjrot_/zjrotf_), swap
-//      (dswp_) or Gram-Schmidt correction (dgsscl_/zgsscl_), with the same
-//      fused multiply-add sequences as the reference, and the max-|.|
-//      tracking that drives the overflow control.
-// Every reduction is a butterfly (all lanes end with identical values), so
-// all decisions are group-uniform.
+//   pair_reduce  exact column norms in (e,f) form and the scaled cosine,
+//                from the pair held in registers (E elements per lane, row
+//                lane + j*B):
+//                  - max-|.| exponent scaling, double-double sums of squares
+//                    and a corrected square root (replaces dnorm2_/znorm2_,
+//                    dnorm2.c:9-14, znorm2.c:9-15);
+//                  - the scaled, compensated dot product (replaces
+//                    ddpscl_/zdpscl_, ddpscl.c:7-75, zdpscl.c:21-116):
+//                    TwoProd + TwoSum per element and a double-double
+//                    reduce-scatter across the lanes.
+//   pair_decide  per-pair scalar work, one lane per pair: the convergence
+//                test, Gram prescaling and action code (dvjsvd.c:349-459)
+//                and the 2x2 EVD (dbjac2i cosine form).
+//   pair_apply   the in-place update: rotation (djrot_/djrotf_/zjrot_/
+//                zjrotf_), swap (dswp_) or Gram-Schmidt correction
+//                (dgsscl_/zgsscl_) with the reference's fused multiply-add
+//                sequences, and the max-|.| tracking of the overflow control.
+//
+// pair_step runs the three phases back to back (used where one group owns
+// one pair); the batched kernel runs pair_decide for all pairs of a step in
+// parallel, one lane per pair, between an LDS-staged reduce and apply.
 #pragma once
 
 #include "vj_dev.hpp"
@@ -25,6 +30,15 @@ namespace vj {
 
 // -------------------------------------------------------------- reductions
 template <typename T> __device__ __forceinline__ T shfl_xor_(T v, int m) { return __shfl_xor(v, m, 64); }
+
+__device__ __forceinline__ double readlane_(double v, int l)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float readlane_(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 
 template <typename T> struct DD {
   T s, c;
@@ -47,6 +61,100 @@ template <typename T> __device__ __forceinline__ void dd_acc(DD<T> &a, T p, T e)
   a.c += (t + e);
 }
 
+template <typename T> __device__ __forceinline__ DD<T> dd_shfl(DD<T> v, int m)
+{
+  return DD<T>{shfl_xor_(v.s, m), shfl_xor_(v.c, m)};
+}
+template <typename T> __device__ __forceinline__ DD<T> dd_sel(bool c, DD<T> a, DD<T> b)
+{
+  return DD<T>{c ? a.s : b.s, c ? a.c : b.c};
+}
+
+// Max of two values over the group.  Wave level: reduce-scatter (lanes < 32
+// finish with max(a), lanes >= 32 with max(b)), then a broadcast.
+template <typename T, int B>
+__device__ __forceinline__ void red_max2(T &a, T &b, double *lds)
+{
+  const int lane = threadIdx.x & 63;
+  const bool h = (lane & 32) != 0;
+  T keep = h ? b : a;
+  const T send = h ? a : b;
+  keep = xmax(keep, shfl_xor_(send, 32));
+#pragma unroll
+  for (int o = 16; o >= 1; o >>= 1)
+    keep = xmax(keep, shfl_xor_(keep, o));
+  a = readlane_(keep, 0);
+  b = readlane_(keep, 32);
+  if constexpr (B > 64) {
+    constexpr int W = B / 64;
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+      lds[2 * w] = (double)a;
+      lds[2 * w + 1] = (double)b;
+    }
+    __syncthreads();
+    T ra = (T)lds[0], rb = (T)lds[1];
+#pragma unroll
+    for (int k = 1; k < W; ++k) {
+      ra = xmax(ra, (T)lds[2 * k]);
+      rb = xmax(rb, (T)lds[2 * k + 1]);
+    }
+    a = ra;
+    b = rb;
+  }
+}
+
+// Sum of four double-double values over the group.  Wave level: the first
+// two butterfly levels scatter the quantities over the lane quarters (each
+// level moves half the data of a plain butterfly), the last four levels
+// reduce one quantity per quarter, and lanes 0/16/32/48 broadcast the totals.
+template <typename A, int B>
+__device__ __forceinline__ void red_dd4(DD<A> &q0, DD<A> &q1, DD<A> &q2, DD<A> &q3, double *lds)
+{
+  const int lane = threadIdx.x & 63;
+  const bool h = (lane & 32) != 0;
+  DD<A> k0 = dd_sel(h, q2, q0), k1 = dd_sel(h, q3, q1);
+  const DD<A> s0 = dd_sel(h, q0, q2), s1 = dd_sel(h, q1, q3);
+  k0 = dd_add(k0, dd_shfl(s0, 32));
+  k1 = dd_add(k1, dd_shfl(s1, 32));
+  const bool g = (lane & 16) != 0;
+  DD<A> k = dd_sel(g, k1, k0);
+  const DD<A> s = dd_sel(g, k0, k1);
+  k = dd_add(k, dd_shfl(s, 16));
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1)
+    k = dd_add(k, dd_shfl(k, o));
+  q0 = DD<A>{readlane_(k.s, 0), readlane_(k.c, 0)};
+  q1 = DD<A>{readlane_(k.s, 16), readlane_(k.c, 16)};
+  q2 = DD<A>{readlane_(k.s, 32), readlane_(k.c, 32)};
+  q3 = DD<A>{readlane_(k.s, 48), readlane_(k.c, 48)};
+  if constexpr (B > 64) {
+    constexpr int W = B / 64;
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+      double *o = lds + 8 * w;
+      o[0] = (double)q0.s; o[1] = (double)q0.c;
+      o[2] = (double)q1.s; o[3] = (double)q1.c;
+      o[4] = (double)q2.s; o[5] = (double)q2.c;
+      o[6] = (double)q3.s; o[7] = (double)q3.c;
+    }
+    __syncthreads();
+    DD<A> r0{(A)lds[0], (A)lds[1]}, r1{(A)lds[2], (A)lds[3]}, r2{(A)lds[4], (A)lds[5]}, r3{(A)lds[6], (A)lds[7]};
+#pragma unroll
+    for (int w2 = 1; w2 < W; ++w2) {
+      const double *o = lds + 8 * w2;
+      r0 = dd_add(r0, DD<A>{(A)o[0], (A)o[1]});
+      r1 = dd_add(r1, DD<A>{(A)o[2], (A)o[3]});
+      r2 = dd_add(r2, DD<A>{(A)o[4], (A)o[5]});
+      r3 = dd_add(r3, DD<A>{(A)o[6], (A)o[7]});
+    }
+    q0 = r0; q1 = r1; q2 = r2; q3 = r3;
+  }
+}
+
+// plain butterfly max over the group (used for the update maxima)
 template <typename T, int B>
 __device__ __forceinline__ T red_max(T v, double *lds)
 {
@@ -64,34 +172,6 @@ __device__ __forceinline__ T red_max(T v, double *lds)
 #pragma unroll
     for (int k = 1; k < W; ++k)
       r = xmax(r, (T)lds[k]);
-    v = r;
-  }
-  return v;
-}
-
-template <typename T, int B>
-__device__ __forceinline__ DD<T> red_dd(DD<T> v, double *lds)
-{
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    DD<T> u;
-    u.s = shfl_xor_(v.s, o);
-    u.c = shfl_xor_(v.c, o);
-    v = dd_add(v, u); // commutative and exact-error: partners agree bitwise
-  }
-  if constexpr (B > 64) {
-    constexpr int W = B / 64;
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) {
-      lds[2 * w] = (double)v.s;
-      lds[2 * w + 1] = (double)v.c;
-    }
-    __syncthreads();
-    DD<T> r{(T)lds[0], (T)lds[1]};
-#pragma unroll
-    for (int k = 1; k < W; ++k)
-      r = dd_add(r, DD<T>{(T)lds[2 * k], (T)lds[2 * k + 1]});
     v = r;
   }
   return v;
@@ -138,26 +218,33 @@ template <typename T> struct Step {
   int vtrack;        // 1: V preset (djrot_ tracking), 0: V = I (djrotf_)
 };
 
-template <typename T, bool CPLX> struct PairOut {
-  int code;          // action code (dvjsvd.c:438-453)
+// Results of pair_reduce: norms of the pair (p = x, q = y) and the cosine
+// a21 = y^H x / (|x| |y|) in the reference's (q, p) order.
+template <typename T> struct PairScal {
+  T ex, fx, ey, fy;
+  T a21r, a21i;
+};
+
+// Results of pair_decide.
+template <typename T> struct PairDec {
+  int code;          // action code (dvjsvd.c:438-453): 3, -3, 2, -2, -1, 1
   int trans;         // 1 if transformed (counted in the sweep total)
   int err;           // 0, or the reference's negative line code
+  T c, at, sat;      // cosine and (complex) tangent of the rotation
+};
+
+template <typename T, bool CPLX> struct PairOut {
+  int code;
+  int trans;
+  int err;
   T tG, tV;          // max |updated G / V element| of this pair
 };
 
-// Returns the reference's action code and updates x,y (registers) in
-// place; V columns are streamed from/to global memory.  All lanes of the
-// group call this with the same p, q.
 template <typename T, bool CPLX, int B, int E>
-__device__ __forceinline__ PairOut<T, CPLX>
-pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
-          T *__restrict__ Vrp, T *__restrict__ Vrq, T *__restrict__ Vip, T *__restrict__ Viq,
-          T &eSp, T &fSp, T &eSq, T &fSq, const Step<T> &S, double *lds, const int lane)
+__device__ __forceinline__ PairScal<T>
+pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E], double *lds)
 {
   typedef typename Acc<T>::A A;
-  PairOut<T, CPLX> out{1, 0, 0, (T)0, (T)0};
-
-  // ---- 1. norms
   T mx = (T)0, my = (T)0;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -168,8 +255,7 @@ pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[
       my = xmax(my, vabs(yi[j]));
     }
   }
-  mx = red_max<T, B>(mx, lds);
-  my = red_max<T, B>(my, lds);
+  red_max2<T, B>(mx, my, lds);
   const int kx = (mx > (T)0) ? (frexp_exp(mx) - 1) : 0;
   const int ky = (my > (T)0) ? (frexp_exp(my) - 1) : 0;
   DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
@@ -212,66 +298,57 @@ pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[
       }
     }
   }
-  if constexpr (Acc<T>::DDACC) {
-    sx = red_dd<A, B>(sx, lds);
-    sy = red_dd<A, B>(sy, lds);
-    dr = red_dd<A, B>(dr, lds);
-    if constexpr (CPLX)
-      di = red_dd<A, B>(di, lds);
-  }
-  else {
-    sx = red_dd<A, B>(sx, lds);
-    sy = red_dd<A, B>(sy, lds);
-    dr = red_dd<A, B>(dr, lds);
-    if constexpr (CPLX)
-      di = red_dd<A, B>(di, lds);
-  }
+  red_dd4<A, B>(sx, sy, dr, di, lds);
   const double rx = (mx > (T)0) ? dd_sqrt(DD<double>{(double)sx.s, (double)sx.c}) : 0.0;
   const double ry = (my > (T)0) ? dd_sqrt(DD<double>{(double)sy.s, (double)sy.c}) : 0.0;
-  T ex, fx, ey, fy;
-  ef_from<T>(rx, kx, ex, fx);
-  ef_from<T>(ry, ky, ey, fy);
-
-  // ---- 2. cosine a21 = y^H x / (|x||y|) of the reference's (q, p) order
-  // (dvjsvd.c:315-319: ddpscl_(m, G_q, G_p, ...)); real part symmetric,
-  // imaginary part of y^H x = -(imag of x^H y).
-  T a21r = (T)0, a21i = (T)0;
+  PairScal<T> o;
+  ef_from<T>(rx, kx, o.ex, o.fx);
+  ef_from<T>(ry, ky, o.ey, o.fy);
+  // cosine of the reference's (q, p) order (dvjsvd.c:315-319): real part
+  // symmetric, imaginary part of y^H x = -(imag of x^H y); the sum is
+  // rounded once and moved by the exact power of two that converts the
+  // max-exponent scaling into the reference's norm-exponent scaling
+  o.a21r = (T)0;
+  o.a21i = (T)0;
   if ((mx > (T)0) && (my > (T)0)) {
-    const int shift = (kx + ky) - (int)(ex + ey);
-    const T de = fx * fy;
-    const T nur = ldexp_((T)((A)dr.s + (A)dr.c), shift);
-    a21r = nur / de;
-    if constexpr (CPLX) {
-      const T nui = ldexp_((T)(-((A)di.s + (A)di.c)), shift);
-      a21i = nui / de;
-    }
+    const int shift = (kx + ky) - (int)(o.ex + o.ey);
+    const T de = o.fx * o.fy;
+    o.a21r = ldexp_((T)((A)dr.s + (A)dr.c), shift) / de;
+    if constexpr (CPLX)
+      o.a21i = ldexp_((T)(-((A)di.s + (A)di.c)), shift) / de;
   }
-  if (!isfinite_(a21r) || !isfinite_(a21i)) {
-    out.err = CPLX ? -345 : -320;
-    return out;
-  }
+  return o;
+}
 
-  // ---- 3. convergence test, Gram prescaling, 2x2 EVD, action code
+// Convergence test, Gram prescaling, 2x2 EVD and action code of one pair
+// (dvjsvd.c:349-459, zvjsvd.c:377-491); per lane.
+template <typename T, bool CPLX>
+__device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T tol, const T gst)
+{
+  PairDec<T> d{1, 0, 0, (T)1, (T)0, (T)0};
+  if (!isfinite_(P.a21r) || !isfinite_(P.a21i)) {
+    d.err = CPLX ? -345 : -320;
+    return d;
+  }
   T aabs;
   if constexpr (CPLX) {
-    const T x_ = vabs(a21r), y_ = vabs(a21i);
+    const T x_ = vabs(P.a21r), y_ = vabs(P.a21i);
     const T m_ = xmin(x_, y_), M_ = xmax(x_, y_);
     T z = xmax(m_ / M_, (T)0);
     z = sqrt_(fma_(z, z, (T)1));
     aabs = z * M_;
   }
   else
-    aabs = vabs(a21r);
-  T c = (T)1, at = (T)0, sat = (T)0;
-  if (S.tol <= aabs) {
-    out.trans = 1;
-    const T e1 = ex, e2 = ey, f1 = fx, f2 = fy;
+    aabs = vabs(P.a21r);
+  if (tol <= aabs) {
+    d.trans = 1;
+    const T e1 = P.ex, e2 = P.ey, f1 = P.fx, f2 = P.fy;
     const bool ngs = eflt(e1, e2, f1, f2);
     T maf = ngs ? f1 : f2;
     T mae = ngs ? e1 : e2;
     const T Maf = ngs ? f2 : f1;
     const T Mae = ngs ? e2 : e1;
-    maf = S.gst * maf;
+    maf = gst * maf;
     mae = mae + getexp(maf);
     maf = getmant(maf);
     const bool cgs = eflt(mae, Mae, maf, Maf);
@@ -282,36 +359,52 @@ pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[
     f21 = getmant(f21);
     const bool c12 = efle(e12, e21, f12, f21);
     const T Ee = c12 ? e21 : e12;
-    const T d = xmin((T)Lim<T>::FIN_EXP - Ee, (T)0);
-    e12 = e12 + d;
-    e21 = e21 + d;
+    const T dd = xmin((T)Lim<T>::FIN_EXP - Ee, (T)0);
+    e12 = e12 + dd;
+    e21 = e21 + dd;
     const T w0 = scalef(f12, e12), w1 = scalef(f21, e21);
     T l1, l2;
-    bool P;
+    bool Pm;
     if constexpr (CPLX)
-      cjac2<T, JAC2_TAN>(w0, w1, scalef(a21r, d), scalef(a21i, d), c, at, sat, l1, l2, P);
+      cjac2<T, JAC2_TAN>(w0, w1, scalef(P.a21r, dd), scalef(P.a21i, dd), d.c, d.at, d.sat, l1, l2, Pm);
     else
-      rjac2<T, JAC2_TAN>(w0, w1, scalef(a21r, d), c, at, l1, l2, P);
-    out.code = (!ngs && cgs) ? 3 : ((ngs && cgs) ? -3 : (P ? -2 : 2));
+      rjac2<T, JAC2_TAN>(w0, w1, scalef(P.a21r, dd), d.c, d.at, l1, l2, Pm);
+    d.code = (!ngs && cgs) ? 3 : ((ngs && cgs) ? -3 : (Pm ? -2 : 2));
   }
   else
-    out.code = (efcmp(ex, fx, ey, fy) < 0) ? -1 : 1;
+    d.code = (efcmp(P.ex, P.fx, P.ey, P.fy) < 0) ? -1 : 1;
+  return d;
+}
 
-  // ---- 4. apply (dvjsvd.c:460-574)
+// In-place update of the pair (dvjsvd.c:460-574): x,y in registers, the V
+// columns streamed.  Writes the pair's (e,f) as the reference leaves them
+// (swapped for codes -1 and -3; the norms of updated columns are recomputed
+// at their next step).  All lanes of the group call this with the same
+// decision.
+template <typename T, bool CPLX, int B, int E>
+__device__ __forceinline__ PairOut<T, CPLX>
+pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
+           T *__restrict__ Vrp, T *__restrict__ Vrq, T *__restrict__ Vip, T *__restrict__ Viq,
+           const PairScal<T> &P, const PairDec<T> &D, const int vtrack, T &eSp, T &fSp, T &eSq, T &fSq,
+           double *lds, const int lane)
+{
+  PairOut<T, CPLX> out{D.code, D.trans, D.err, (T)0, (T)0};
+  if (D.err)
+    return out;
+  const T ex = P.ex, fx = P.fx, ey = P.ey, fy = P.fy;
+  const T c = D.c, at = D.at, sat = D.sat;
   T tG = (T)0, tV = (T)0;
-  const int code = out.code;
+  const int code = D.code;
   if ((code == 2) || (code == -2)) {
     const bool perm = (code < 0);
     // G pair in registers: djrot_/zjrot_ with length +-m; a complex
     // rotation with sat == 0 is two real rotations (zjrot.c:26-35)
     if ((at == (T)0) && (!CPLX || (sat == (T)0))) {
-      if (c == (T)1) {
-        if (perm) {
+      if ((c == (T)1) && perm) {
 #pragma unroll
-          for (int j = 0; j < E; ++j) {
-            T t = xr[j]; xr[j] = yr[j]; yr[j] = t;
-            if constexpr (CPLX) { t = xi[j]; xi[j] = yi[j]; yi[j] = t; }
-          }
+        for (int j = 0; j < E; ++j) {
+          T t = xr[j]; xr[j] = yr[j]; yr[j] = t;
+          if constexpr (CPLX) { t = xi[j]; xi[j] = yi[j]; yi[j] = t; }
         }
       }
     }
@@ -397,20 +490,20 @@ pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[
           else { Vrp[i] = ar; Vip[i] = ai; Vrq[i] = br; Viq[i] = bi; }
         }
       }
-      if (S.vtrack)
+      if (vtrack)
         tV = red_max<T, B>(tV, lds);
       else
         tV = (T)0;
     }
     if (!isfinite_(tG)) { out.err = CPLX ? -630 : -580; return out; }
     if (!isfinite_(tV)) { out.err = CPLX ? -638 : -588; return out; }
-    // norms of rotated columns are recomputed at the next step
     eSp = ex; fSp = fx; eSq = ey; fSq = fy;
   }
   else if ((code == 3) || (code == -3)) {
     // dgsscl_/zgsscl_ (dgsscl.c:5-85, zgsscl.c:5-119) with t = a21
     const bool perm = (code < 0);
     const T x_e = -ex, y_e = -ey;
+    const T a21r = P.a21r, a21i = P.a21i;
     if (!perm) {
       const T fy_x = fy / fx;
       const T tfr = -a21r * fy_x;
@@ -485,6 +578,19 @@ pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[
   out.tG = tG;
   out.tV = tV;
   return out;
+}
+
+// The three phases back to back, for a group that owns one pair.
+template <typename T, bool CPLX, int B, int E>
+__device__ __forceinline__ PairOut<T, CPLX>
+pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
+          T *__restrict__ Vrp, T *__restrict__ Vrq, T *__restrict__ Vip, T *__restrict__ Viq,
+          T &eSp, T &fSp, T &eSq, T &fSq, const Step<T> &S, double *lds, const int lane)
+{
+  (void)m;
+  const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, lds);
+  const PairDec<T> D = pair_decide<T, CPLX>(P, S.tol, S.gst);
+  return pair_apply<T, CPLX, B, E>(n, xr, yr, xi, yi, Vrp, Vrq, Vip, Viq, P, D, S.vtrack, eSp, fSp, eSq, fSq, lds, lane);
 }
 
 } // namespace vj

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
                                              const unsigned stp, const unsigned swp, const T tol, const T gst,
                                              const int vtrack, PlainState<T> *__restrict__ st)
 {
-  __shared__ double lds[2 * (B / 64) + 2];
+  __shared__ double lds[8 * (B / 64) + 2];
   __shared__ int last;
   if (*(volatile int *)&st->done)
     return;
