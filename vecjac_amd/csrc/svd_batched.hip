@@ -23,15 +23,18 @@
 
 namespace vj {
 
-// Column pair (p, q) of an m-row matrix into registers, row lane + 64 j,
-// scaled by 2^sG when a rescaling is pending.
+// Column pair (p, q) of an m-row matrix into registers, row lane + 64 j for
+// the nE = ceil(m / 64) chunks in use, scaled by 2^sG when a rescaling is
+// pending.
 template <typename T, bool CPLX, int E>
-__device__ __forceinline__ void load_pair(const T *Gr, const T *Gi, const int m, const int p, const int q, const int sG,
-                                          const int lane, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E])
+__device__ __forceinline__ void load_pair(const T *Gr, const T *Gi, const int m, const int nE, const int p, const int q,
+                                          const int sG, const int lane, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E])
 {
   const T *const Gp = Gr + (long long)p * m, *const Gq = Gr + (long long)q * m;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
     const int i = lane + j * 64;
     const bool in = (i < m);
     xr[j] = in ? Gp[i] : (T)0;
@@ -42,7 +45,12 @@ __device__ __forceinline__ void load_pair(const T *Gr, const T *Gi, const int m,
     }
     else
       xi[j] = yi[j] = (T)0;
-    if (sG) {
+  }
+  if (sG) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      if (j >= nE)
+        break;
       xr[j] = ldexp_(xr[j], sG);
       yr[j] = ldexp_(yr[j], sG);
       if constexpr (CPLX) {
@@ -54,12 +62,14 @@ __device__ __forceinline__ void load_pair(const T *Gr, const T *Gi, const int m,
 }
 
 template <typename T, bool CPLX, int E>
-__device__ __forceinline__ void store_pair(T *Gr, T *Gi, const int m, const int p, const int q, const int lane,
+__device__ __forceinline__ void store_pair(T *Gr, T *Gi, const int m, const int nE, const int p, const int q, const int lane,
                                            const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E])
 {
   T *const Gp = Gr + (long long)p * m, *const Gq = Gr + (long long)q * m;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
     const int i = lane + j * 64;
     if (i < m) {
       Gp[i] = xr[j];
@@ -82,7 +92,7 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
   __shared__ double s_max[2][W][2];            // per-wave (tG, tV) maxima, by step parity
   __shared__ unsigned long long s_cnt[2][W];   // per-wave transformation counts, by step parity
   __shared__ int s_err[2][W];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int rot = CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R;
 
   for (;;) {
@@ -90,7 +100,7 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
     if (threadIdx.x == 0)
       s_b = (int)atomicAdd(a.queue, 1u);
     __syncthreads();
-    const int bi = s_b;
+    const int bi = __builtin_amdgcn_readfirstlane(s_b);
     if (bi >= a.nb)
       break;
     const int b = a.order[bi];
@@ -103,6 +113,7 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
     T *const fS = a.fS + a.offS[b];
     const unsigned *const js = a.tables + a.tab_off[n];
     const unsigned stp = (unsigned)(n - 1);
+    const int nE = (m + 63) >> 6;
 
     // ---- setup (dvjsvd.c:78-184): max |G|, V := I, eS/fS, initial scale
     T mxv = (T)0;
@@ -167,14 +178,28 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
         for (unsigned st = 0; st < stp; ++st) {
           const unsigned *const row = js + (size_t)st * n;
           const int npairs = n >> 1;
-          // ---- phase A: norms and cosines of this wave's pairs -> LDS
-          for (int k = wave; k < npairs; k += W) {
-            const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
+          // ---- phase A: norms and cosines of this wave's pairs -> LDS; the
+          // next pair's columns are in flight while this one is reduced
+          {
             T xr[E], yr[E], xi[E], yi[E];
-            load_pair<T, CPLX, E>(Gr, Gi, m, p, q, S.sG, lane, xr, yr, xi, yi);
-            const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, nullptr);
-            if (lane == 0)
-              s_scal[k] = P;
+            if (wave < npairs)
+              load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)row[2 * wave], (int)row[2 * wave + 1], S.sG, lane, xr, yr, xi, yi);
+            for (int k = wave; k < npairs; k += W) {
+              T nxr[E], nyr[E], nxi[E], nyi[E];
+              const int kn = k + W;
+              if (kn < npairs)
+                load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)row[2 * kn], (int)row[2 * kn + 1], S.sG, lane, nxr, nyr, nxi, nyi);
+              const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, nullptr, nE);
+              if (lane == 0)
+                s_scal[k] = P;
+#pragma unroll
+              for (int j = 0; j < E; ++j) {
+                xr[j] = nxr[j];
+                yr[j] = nyr[j];
+                xi[j] = nxi[j];
+                yi[j] = nyi[j];
+              }
+            }
           }
           __syncthreads();
           // ---- phase B: decisions and 2x2 EVDs of all pairs, one lane per pair
@@ -205,10 +230,10 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
             PairOut<T, CPLX> o;
             if ((D.code != 1) || S.sG || D.err) {
               T xr[E], yr[E], xi[E], yi[E];
-              load_pair<T, CPLX, E>(Gr, Gi, m, p, q, S.sG, lane, xr, yr, xi, yi);
-              o = pair_apply<T, CPLX, B, E>(n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane);
+              load_pair<T, CPLX, E>(Gr, Gi, m, nE, p, q, S.sG, lane, xr, yr, xi, yi);
+              o = pair_apply<T, CPLX, B, E>(n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane, nE);
               if (!o.err)
-                store_pair<T, CPLX, E>(Gr, Gi, m, p, q, lane, xr, yr, xi, yi);
+                store_pair<T, CPLX, E>(Gr, Gi, m, nE, p, q, lane, xr, yr, xi, yi);
             }
             else {
               o = PairOut<T, CPLX>{1, 0, 0, (T)0, (T)0};

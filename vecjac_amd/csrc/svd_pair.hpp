@@ -29,7 +29,65 @@
 namespace vj {
 
 // -------------------------------------------------------------- reductions
-template <typename T> __device__ __forceinline__ T shfl_xor_(T v, int m) { return __shfl_xor(v, m, 64); }
+// Wave-level exchanges without an LDS round trip: DPP moves inside rows of
+// 16 lanes (mirror, half mirror, quad xor 2, quad xor 1: after the four
+// levels every lane of a row holds the row total) and the gfx950 permlane
+// swaps between the wave halves (32) and neighbouring rows (16):
+//   swap32(a, b): a := [a lanes 0-31, b lanes 0-31], b := [a lanes 32-63, b lanes 32-63]
+//   swap16(a, b): a := rows (a0, b0, a2, b2),       b := rows (a1, b1, a3, b3)
+// so a + b after a swap is a reduce-scatter step of two quantities.
+constexpr int DPP_MIRROR = 0x140, DPP_HMIRROR = 0x141, DPP_XOR2 = 0x4E, DPP_XOR1 = 0xB1;
+
+template <int CTRL> __device__ __forceinline__ double dpp_(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ void swap32_(unsigned &a, unsigned &b)
+{
+  const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+  a = r[0];
+  b = r[1];
+}
+__device__ __forceinline__ void swap16_(unsigned &a, unsigned &b)
+{
+  const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+  a = r[0];
+  b = r[1];
+}
+template <bool H32> __device__ __forceinline__ void swap_(double &a, double &b)
+{
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  unsigned al = (unsigned)ua, ah = (unsigned)(ua >> 32), bl = (unsigned)ub, bh = (unsigned)(ub >> 32);
+  if constexpr (H32) {
+    swap32_(al, bl);
+    swap32_(ah, bh);
+  }
+  else {
+    swap16_(al, bl);
+    swap16_(ah, bh);
+  }
+  a = __longlong_as_double((long long)(((unsigned long long)ah << 32) | al));
+  b = __longlong_as_double((long long)(((unsigned long long)bh << 32) | bl));
+}
+template <bool H32> __device__ __forceinline__ void swap_(float &a, float &b)
+{
+  unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
+  if constexpr (H32)
+    swap32_(ua, ub);
+  else
+    swap16_(ua, ub);
+  a = __uint_as_float(ua);
+  b = __uint_as_float(ub);
+}
 
 __device__ __forceinline__ double readlane_(double v, int l)
 {
@@ -61,32 +119,49 @@ template <typename T> __device__ __forceinline__ void dd_acc(DD<T> &a, T p, T e)
   a.c += (t + e);
 }
 
-template <typename T> __device__ __forceinline__ DD<T> dd_shfl(DD<T> v, int m)
+template <int CTRL, typename T> __device__ __forceinline__ DD<T> dd_dpp(DD<T> v)
 {
-  return DD<T>{shfl_xor_(v.s, m), shfl_xor_(v.c, m)};
+  return DD<T>{dpp_<CTRL>(v.s), dpp_<CTRL>(v.c)};
 }
-template <typename T> __device__ __forceinline__ DD<T> dd_sel(bool c, DD<T> a, DD<T> b)
+template <bool H32, typename T> __device__ __forceinline__ DD<T> dd_swapadd(DD<T> a, DD<T> b)
 {
-  return DD<T>{c ? a.s : b.s, c ? a.c : b.c};
+  swap_<H32>(a.s, b.s);
+  swap_<H32>(a.c, b.c);
+  return dd_add(a, b);
 }
 
-// Max of two values over the group.  Wave level: reduce-scatter (lanes < 32
-// finish with max(a), lanes >= 32 with max(b)), then a broadcast.
+// row totals (every lane of a row of 16) of a max / a double-double sum
+template <typename T> __device__ __forceinline__ T row_max(T v)
+{
+  v = xmax(v, dpp_<DPP_MIRROR>(v));
+  v = xmax(v, dpp_<DPP_HMIRROR>(v));
+  v = xmax(v, dpp_<DPP_XOR2>(v));
+  v = xmax(v, dpp_<DPP_XOR1>(v));
+  return v;
+}
+template <typename T> __device__ __forceinline__ DD<T> row_sum(DD<T> v)
+{
+  v = dd_add(v, dd_dpp<DPP_MIRROR>(v));
+  v = dd_add(v, dd_dpp<DPP_HMIRROR>(v));
+  v = dd_add(v, dd_dpp<DPP_XOR2>(v));
+  v = dd_add(v, dd_dpp<DPP_XOR1>(v));
+  return v;
+}
+
+// Max of two values over the group.  Wave level: one swap32 scatters a to
+// the low half and b to the high half, the rows reduce, and lanes 0/16
+// (a) and 32/48 (b) are combined.
 template <typename T, int B>
 __device__ __forceinline__ void red_max2(T &a, T &b, double *lds)
 {
-  const int lane = threadIdx.x & 63;
-  const bool h = (lane & 32) != 0;
-  T keep = h ? b : a;
-  const T send = h ? a : b;
-  keep = xmax(keep, shfl_xor_(send, 32));
-#pragma unroll
-  for (int o = 16; o >= 1; o >>= 1)
-    keep = xmax(keep, shfl_xor_(keep, o));
-  a = readlane_(keep, 0);
-  b = readlane_(keep, 32);
+  T lo = a, hi = b;
+  swap_<true>(lo, hi);
+  const T k = row_max(xmax(lo, hi));
+  a = xmax(readlane_(k, 0), readlane_(k, 16));
+  b = xmax(readlane_(k, 32), readlane_(k, 48));
   if constexpr (B > 64) {
     constexpr int W = B / 64;
+    const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) {
@@ -96,41 +171,32 @@ __device__ __forceinline__ void red_max2(T &a, T &b, double *lds)
     __syncthreads();
     T ra = (T)lds[0], rb = (T)lds[1];
 #pragma unroll
-    for (int k = 1; k < W; ++k) {
-      ra = xmax(ra, (T)lds[2 * k]);
-      rb = xmax(rb, (T)lds[2 * k + 1]);
+    for (int k2 = 1; k2 < W; ++k2) {
+      ra = xmax(ra, (T)lds[2 * k2]);
+      rb = xmax(rb, (T)lds[2 * k2 + 1]);
     }
     a = ra;
     b = rb;
   }
 }
 
-// Sum of four double-double values over the group.  Wave level: the first
-// two butterfly levels scatter the quantities over the lane quarters (each
-// level moves half the data of a plain butterfly), the last four levels
-// reduce one quantity per quarter, and lanes 0/16/32/48 broadcast the totals.
+// Sum of four double-double values over the group.  Wave level: swap32
+// scatters (q0, q2) and (q1, q3) over the halves, swap16 scatters the two
+// results over the rows (row r then holds a partial sum of q_r), the rows
+// reduce with DPP and lanes 0/16/32/48 hold q0..q3.
 template <typename A, int B>
 __device__ __forceinline__ void red_dd4(DD<A> &q0, DD<A> &q1, DD<A> &q2, DD<A> &q3, double *lds)
 {
-  const int lane = threadIdx.x & 63;
-  const bool h = (lane & 32) != 0;
-  DD<A> k0 = dd_sel(h, q2, q0), k1 = dd_sel(h, q3, q1);
-  const DD<A> s0 = dd_sel(h, q0, q2), s1 = dd_sel(h, q1, q3);
-  k0 = dd_add(k0, dd_shfl(s0, 32));
-  k1 = dd_add(k1, dd_shfl(s1, 32));
-  const bool g = (lane & 16) != 0;
-  DD<A> k = dd_sel(g, k1, k0);
-  const DD<A> s = dd_sel(g, k0, k1);
-  k = dd_add(k, dd_shfl(s, 16));
-#pragma unroll
-  for (int o = 8; o >= 1; o >>= 1)
-    k = dd_add(k, dd_shfl(k, o));
+  const DD<A> k0 = dd_swapadd<true>(q0, q2);
+  const DD<A> k1 = dd_swapadd<true>(q1, q3);
+  const DD<A> k = row_sum(dd_swapadd<false>(k0, k1));
   q0 = DD<A>{readlane_(k.s, 0), readlane_(k.c, 0)};
   q1 = DD<A>{readlane_(k.s, 16), readlane_(k.c, 16)};
   q2 = DD<A>{readlane_(k.s, 32), readlane_(k.c, 32)};
   q3 = DD<A>{readlane_(k.s, 48), readlane_(k.c, 48)};
   if constexpr (B > 64) {
     constexpr int W = B / 64;
+    const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) {
@@ -154,13 +220,12 @@ __device__ __forceinline__ void red_dd4(DD<A> &q0, DD<A> &q1, DD<A> &q2, DD<A> &
   }
 }
 
-// plain butterfly max over the group (used for the update maxima)
+// max over the group (used for the update maxima)
 template <typename T, int B>
 __device__ __forceinline__ T red_max(T v, double *lds)
 {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1)
-    v = xmax(v, shfl_xor_(v, o));
+  const T k = row_max(v);
+  v = xmax(xmax(readlane_(k, 0), readlane_(k, 16)), xmax(readlane_(k, 32), readlane_(k, 48)));
   if constexpr (B > 64) {
     constexpr int W = B / 64;
     const int w = threadIdx.x >> 6;
@@ -170,8 +235,8 @@ __device__ __forceinline__ T red_max(T v, double *lds)
     __syncthreads();
     T r = (T)lds[0];
 #pragma unroll
-    for (int k = 1; k < W; ++k)
-      r = xmax(r, (T)lds[k]);
+    for (int k2 = 1; k2 < W; ++k2)
+      r = xmax(r, (T)lds[k2]);
     v = r;
   }
   return v;
@@ -242,17 +307,21 @@ template <typename T, bool CPLX> struct PairOut {
 
 template <typename T, bool CPLX, int B, int E>
 __device__ __forceinline__ PairScal<T>
-pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E], double *lds)
+pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E], double *lds, const int nE = E)
 {
   typedef typename Acc<T>::A A;
+  // G is finite whenever a pair is reduced (checked at set-up and after
+  // every update), so the IEEE maximum equals the ordered maximum here
   T mx = (T)0, my = (T)0;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    mx = xmax(mx, vabs(xr[j]));
-    my = xmax(my, vabs(yr[j]));
-    if constexpr (CPLX) {
-      mx = xmax(mx, vabs(xi[j]));
-      my = xmax(my, vabs(yi[j]));
+    if (j < nE) {
+      mx = __builtin_fmax(mx, vabs(xr[j]));
+      my = __builtin_fmax(my, vabs(yr[j]));
+      if constexpr (CPLX) {
+        mx = __builtin_fmax(mx, vabs(xi[j]));
+        my = __builtin_fmax(my, vabs(yi[j]));
+      }
     }
   }
   red_max2<T, B>(mx, my, lds);
@@ -261,6 +330,8 @@ pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[
   DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
 #pragma unroll
   for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
     const A a = ldexp_((A)xr[j], -kx), b = ldexp_((A)yr[j], -ky);
     if constexpr (Acc<T>::DDACC) {
       A p = a * a;
@@ -386,7 +457,7 @@ __device__ __forceinline__ PairOut<T, CPLX>
 pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
            T *__restrict__ Vrp, T *__restrict__ Vrq, T *__restrict__ Vip, T *__restrict__ Viq,
            const PairScal<T> &P, const PairDec<T> &D, const int vtrack, T &eSp, T &fSp, T &eSq, T &fSq,
-           double *lds, const int lane)
+           double *lds, const int lane, const int nE = E)
 {
   PairOut<T, CPLX> out{D.code, D.trans, D.err, (T)0, (T)0};
   if (D.err)
@@ -403,6 +474,8 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
       if ((c == (T)1) && perm) {
 #pragma unroll
         for (int j = 0; j < E; ++j) {
+          if (j >= nE)
+            break;
           T t = xr[j]; xr[j] = yr[j]; yr[j] = t;
           if constexpr (CPLX) { t = xi[j]; xi[j] = yi[j]; yi[j] = t; }
         }
@@ -412,6 +485,8 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
       const bool c1 = (c == (T)1);
 #pragma unroll
       for (int j = 0; j < E; ++j) {
+        if (j >= nE)
+          break;
         T a = perm ? fma_(-xr[j], at, yr[j]) : fma_(yr[j], at, xr[j]);
         T b = perm ? fma_(yr[j], at, xr[j]) : fma_(-xr[j], at, yr[j]);
         if (!c1) { a = a * c; b = b * c; }
@@ -434,6 +509,8 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
         const bool c1 = (c == (T)1);
 #pragma unroll
         for (int j = 0; j < E; ++j) {
+          if (j >= nE)
+            break;
           const T x_r = xr[j], x_i = xi[j], y_r = yr[j], y_i = yi[j];
           T ar = fma_(y_r, at, fma_(-y_i, sat, x_r));
           T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
@@ -510,6 +587,8 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
       const T tfi = a21i * fy_x;
 #pragma unroll
       for (int j = 0; j < E; ++j) {
+        if (j >= nE)
+          break;
         const T xs = scalef_fin(xr[j], x_e);
         const T ys = scalef_fin(yr[j], y_e);
         if constexpr (CPLX) {
@@ -533,6 +612,8 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
       const T tfi = -a21i * fx_y;
 #pragma unroll
       for (int j = 0; j < E; ++j) {
+        if (j >= nE)
+          break;
         const T xs = scalef_fin(xr[j], x_e);
         const T ys = scalef_fin(yr[j], y_e);
         if constexpr (CPLX) {
@@ -563,6 +644,8 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
   else if (code == -1) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
+      if (j >= nE)
+        break;
       T t = xr[j]; xr[j] = yr[j]; yr[j] = t;
       if constexpr (CPLX) { t = xi[j]; xi[j] = yi[j]; yi[j] = t; }
     }
