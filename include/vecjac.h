@@ -111,8 +111,8 @@ vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, floa
 
 /* Plan for a batch of nb independent SVDs solved in one persistent launch.
  * kind: 0 = FP64 real, 1 = FP64 complex (split re/im), 2 = FP32 real,
- * 3 = FP32 complex.  Host descriptors: m[b], n[b] (n even, 2 <= n <= m <=
- * 1024), element offsets offG[b] of G_b (ld m[b]), offV[b] of V_b (ld n[b])
+ * 3 = FP32 complex.  Host descriptors: m[b], n[b] (n even, 2 <= n <= 128,
+ * n <= m <= 1024), element offsets offG[b] of G_b (ld m[b]), offV[b] of V_b (ld n[b])
  * and offS[b] of eS_b/fS_b in the packed device buffers.  waves: waves per
  * workgroup (1, 2, 4 or 8; 0 picks from the median width).  Returns NULL on
  * failure with *status < 0. */

@@ -1,7 +1,8 @@
 // C-ABI of the batched SVD (no reference counterpart): a plan holds the
-// shape-dependent metadata on the device (descriptors, cost order, strategy
-// tables), so that a run is a single kernel launch on resident data and can
-// be captured in a HIP graph.
+// shape-dependent metadata on the device (descriptors and the cost order),
+// so that a run is a single kernel launch on resident data and can be
+// captured in a HIP graph.  The kernel computes the round-robin pairs of
+// each step itself (the pairs of pjs(2, n)).
 #include "svd_batched.hpp"
 #include "svd_common.hpp"
 #include "vecjac.h"
@@ -23,9 +24,8 @@ struct BPlan {
   int waves = 4;
   unsigned swp = 999u;
   int dev = 0;
-  long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each), tab_off (nmax+1)
+  long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each)
   int *d_order = nullptr;
-  unsigned *d_tables = nullptr;
   unsigned *d_queue = nullptr;
 };
 
@@ -37,6 +37,16 @@ int pick_waves(const std::vector<long long> &n)
   std::nth_element(s.begin(), s.begin() + s.size() / 2, s.end());
   const long long med = s[s.size() / 2];
   return (med <= 24) ? 1 : ((med <= 48) ? 2 : 4);
+}
+
+// Eight waves per workgroup leave 256 registers per lane; plans whose
+// (double-buffered) column pairs would spill at that size run four waves.
+int cap_waves(int kind, int max_m, int waves)
+{
+  const int e = (max_m <= 64) ? 1 : (max_m <= 128) ? 2 : (max_m <= 256) ? 4 : (max_m <= 512) ? 8 : 16;
+  const int bytes = e * ((kind == 0 || kind == 1) ? 8 : 4) * ((kind & 1) ? 2 : 1);
+  const int limit = (kind & 1) ? 32 : 64;
+  return (waves > 4 && bytes > limit) ? 4 : waves;
 }
 
 template <typename T, bool CPLX>
@@ -51,9 +61,7 @@ int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, v
   a.offG = P->d_meta + 2 * nb;
   a.offV = P->d_meta + 3 * nb;
   a.offS = P->d_meta + 4 * nb;
-  a.tab_off = P->d_meta + 5 * nb;
   a.order = P->d_order;
-  a.tables = P->d_tables;
   a.Gr = (T *)Gr;
   a.Gi = (T *)Gi;
   a.Vr = (T *)Vr;
@@ -84,15 +92,14 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
     return nullptr;
   }
   std::vector<long long> hm(m, m + nb), hn(n, n + nb);
-  long long nmax = 0;
   int max_m = 0;
   for (vj_fint b = 0; b < nb; ++b) {
-    if (hn[b] < 2 || (hn[b] & 1) || hm[b] < hn[b] || hm[b] > 1024 || offG[b] < 0 || offV[b] < 0 || offS[b] < 0) {
-      *st = set_error(-2, "matrix %lld: need even n >= 2, n <= m <= 1024 and non-negative offsets (m=%lld n=%lld)",
+    if (hn[b] < 2 || (hn[b] & 1) || hn[b] > 128 || hm[b] < hn[b] || hm[b] > 1024 || offG[b] < 0 || offV[b] < 0 ||
+        offS[b] < 0) {
+      *st = set_error(-2, "matrix %lld: need even 2 <= n <= 128, n <= m <= 1024 and non-negative offsets (m=%lld n=%lld)",
                       (long long)b, hm[b], hn[b]);
       return nullptr;
     }
-    nmax = std::max(nmax, hn[b]);
     max_m = std::max(max_m, (int)hm[b]);
   }
   BPlan *P = new BPlan();
@@ -100,7 +107,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   P->nb = (int)nb;
   P->max_m = max_m;
   P->swp = swp;
-  P->waves = (waves > 0) ? waves : (nb ? pick_waves(hn) : 4);
+  P->waves = cap_waves(kind, max_m, (waves > 0) ? waves : (nb ? pick_waves(hn) : 4));
   (void)hipGetDevice(&P->dev);
   // processing order: decreasing m n^2 (longest first)
   std::vector<int> order((size_t)nb);
@@ -108,45 +115,23 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
     return hm[x] * hn[x] * hn[x] > hm[y] * hn[y] * hn[y];
   });
-  // ME tables of every distinct width
-  std::vector<long long> tab_off((size_t)nmax + 1, 0);
-  std::vector<unsigned> tables;
-  std::map<long long, bool> widths;
-  for (long long w : hn)
-    widths[w] = true;
-  for (auto &kv : widths) {
-    unsigned stp = 0;
-    unsigned *t = pjs(2l, (unsigned)kv.first, &stp);
-    if (!t) {
-      delete P;
-      *st = set_error(-2, "no strategy table for n = %lld", kv.first);
-      return nullptr;
-    }
-    tab_off[(size_t)kv.first] = (long long)tables.size();
-    tables.insert(tables.end(), t, t + (size_t)stp * (size_t)kv.first);
-    free(t);
-  }
   std::vector<long long> meta;
-  meta.reserve((size_t)(5 * nb) + tab_off.size());
+  meta.reserve((size_t)(5 * nb));
   meta.insert(meta.end(), hm.begin(), hm.end());
   meta.insert(meta.end(), hn.begin(), hn.end());
   meta.insert(meta.end(), offG, offG + nb);
   meta.insert(meta.end(), offV, offV + nb);
   meta.insert(meta.end(), offS, offS + nb);
-  meta.insert(meta.end(), tab_off.begin(), tab_off.end());
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_meta, std::max<size_t>(1, meta.size()) * sizeof(long long));
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_order, std::max<size_t>(1, order.size()) * sizeof(int));
-  if (e == hipSuccess) e = hipMalloc((void **)&P->d_tables, std::max<size_t>(1, tables.size()) * sizeof(unsigned));
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, sizeof(unsigned));
   if (e == hipSuccess && !meta.empty()) e = hipMemcpy(P->d_meta, meta.data(), meta.size() * sizeof(long long), hipMemcpyHostToDevice);
   if (e == hipSuccess && !order.empty()) e = hipMemcpy(P->d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !tables.empty()) e = hipMemcpy(P->d_tables, tables.data(), tables.size() * sizeof(unsigned), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     *st = set_error(VJ_EHIP - (int)e, "plan allocation failed: %s", hipGetErrorString(e));
     (void)hipFree(P->d_meta);
     (void)hipFree(P->d_order);
-    (void)hipFree(P->d_tables);
     (void)hipFree(P->d_queue);
     delete P;
     return nullptr;
@@ -178,7 +163,6 @@ void vjh_bplan_destroy(void *plan)
     return;
   (void)hipFree(P->d_meta);
   (void)hipFree(P->d_order);
-  (void)hipFree(P->d_tables);
   (void)hipFree(P->d_queue);
   delete P;
 }

@@ -13,6 +13,8 @@
 //        the n/2 "diameter" pairs {i, i+n/2} meet twice per sweep.
 // Every row is a permutation of 0..n-1 (needed for race-free column
 // updates); each pair is written with its smaller index first.
+#include "pjs.hpp"
+
 #include <stdint.h>
 #include <stdlib.h>
 
@@ -26,12 +28,10 @@ inline void put(unsigned *r, unsigned k, unsigned a, unsigned b)
 
 void round_robin(unsigned n, unsigned *tab)
 {
-  const unsigned m = n - 1u; // odd
-  for (unsigned s = 0u; s < m; ++s) {
+  for (unsigned s = 0u; s + 1u < n; ++s) {
     unsigned *const r = tab + (size_t)s * n;
-    put(r, 0u, s, m);
-    for (unsigned k = 1u; k < (n >> 1u); ++k)
-      put(r, k, (s + k) % m, (s + m - k) % m);
+    for (unsigned k = 0u; k < (n >> 1u); ++k)
+      vj::rr_pair(n, s, k, r[2u * k], r[2u * k + 1u]);
   }
 }
 

@@ -20,37 +20,54 @@
 // the same rescaling and convergence decisions; no host round trip.
 #include "svd_common.hpp"
 #include "svd_batched.hpp"
+#include "pjs.hpp"
 
 namespace vj {
 
-// Column pair (p, q) of an m-row matrix into registers, row lane + 64 j for
-// the nE = ceil(m / 64) chunks in use, scaled by 2^sG when a rescaling is
-// pending.
+// Column pair (p, q) of an m-row matrix into registers, row lane + 64 j.
+// All E chunks are loaded unconditionally (rows past m read row m - 1 and
+// are zeroed at first use): a fixed number of loads per pair lets the
+// compiler wait for exactly the loads of the pair in use and not for the
+// prefetched ones behind them.
 template <typename T, bool CPLX, int E>
 __device__ __forceinline__ void load_pair(const T *Gr, const T *Gi, const int m, const int nE, const int p, const int q,
-                                          const int sG, const int lane, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E])
+                                          const int lane, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E])
 {
-  const T *const Gp = Gr + (long long)p * m, *const Gq = Gr + (long long)q * m;
+  (void)nE;
+  const long long op = (long long)p * m, oq = (long long)q * m;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int i = lane + j * 64;
+    const int ic = (i < m) ? i : (m - 1);
+    xr[j] = Gr[op + ic];
+    yr[j] = Gr[oq + ic];
+    if constexpr (CPLX) {
+      xi[j] = Gi[op + ic];
+      yi[j] = Gi[oq + ic];
+    }
+  }
+}
+
+// Zero the rows past m and apply a pending 2^sG rescaling (at first use of
+// a prefetched pair).
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ void ready_pair(const int m, const int nE, const int sG, const int lane, T (&xr)[E], T (&yr)[E],
+                                           T (&xi)[E], T (&yi)[E])
+{
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     if (j >= nE)
       break;
-    const int i = lane + j * 64;
-    const bool in = (i < m);
-    xr[j] = in ? Gp[i] : (T)0;
-    yr[j] = in ? Gq[i] : (T)0;
+    const bool in = (lane + j * 64) < m;
+    xr[j] = in ? xr[j] : (T)0;
+    yr[j] = in ? yr[j] : (T)0;
     if constexpr (CPLX) {
-      xi[j] = in ? Gi[(long long)p * m + i] : (T)0;
-      yi[j] = in ? Gi[(long long)q * m + i] : (T)0;
+      xi[j] = in ? xi[j] : (T)0;
+      yi[j] = in ? yi[j] : (T)0;
     }
     else
       xi[j] = yi[j] = (T)0;
-  }
-  if (sG) {
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      if (j >= nE)
-        break;
+    if (sG) {
       xr[j] = ldexp_(xr[j], sG);
       yr[j] = ldexp_(yr[j], sG);
       if constexpr (CPLX) {
@@ -82,12 +99,126 @@ __device__ __forceinline__ void store_pair(T *Gr, T *Gi, const int m, const int 
   }
 }
 
+// V column pair (p, q) of an n x n matrix (n <= 128) into registers, row
+// lane + 64 j; unconditional loads as in load_pair.
+template <typename T, bool CPLX>
+__device__ __forceinline__ void load_vpair(const T *Vr, const T *Vi, const int n, const int p, const int q, const int lane,
+                                           T (&vx)[2], T (&vy)[2], T (&vxi)[2], T (&vyi)[2])
+{
+  const long long op = (long long)p * n, oq = (long long)q * n;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int i = lane + 64 * j;
+    const int ic = (i < n) ? i : (n - 1);
+    vx[j] = Vr[op + ic];
+    vy[j] = Vr[oq + ic];
+    if constexpr (CPLX) {
+      vxi[j] = Vi[op + ic];
+      vyi[j] = Vi[oq + ic];
+    }
+    else
+      vxi[j] = vyi[j] = (T)0;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void scale_vpair(const int sV, T (&vx)[2], T (&vy)[2], T (&vxi)[2], T (&vyi)[2])
+{
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    vx[j] = ldexp_(vx[j], sV);
+    vy[j] = ldexp_(vy[j], sV);
+    vxi[j] = ldexp_(vxi[j], sV);
+    vyi[j] = ldexp_(vyi[j], sV);
+  }
+}
+
+template <typename T, bool CPLX>
+__device__ __forceinline__ void store_vpair(T *Vr, T *Vi, const int n, const int p, const int q, const int lane,
+                                            const T (&vx)[2], const T (&vy)[2], const T (&vxi)[2], const T (&vyi)[2])
+{
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int i = lane + 64 * j;
+    if (i < n) {
+      Vr[(long long)p * n + i] = vx[j];
+      Vr[(long long)q * n + i] = vy[j];
+      if constexpr (CPLX) {
+        Vi[(long long)p * n + i] = vxi[j];
+        Vi[(long long)q * n + i] = vyi[j];
+      }
+    }
+  }
+}
+
+// The V part of pair_apply for V held in registers: the same swap or
+// rotation, with the same fused multiply-adds (V := I at the start, so no
+// max-|.| tracking).
+template <typename T, bool CPLX>
+__device__ __forceinline__ void v_update(const PairDec<T> &D, T (&vx)[2], T (&vy)[2], T (&vxi)[2], T (&vyi)[2])
+{
+  const int code = D.code;
+  const T c = D.c, at = D.at, sat = D.sat;
+  bool swap = (code == -1) || (code == -3);
+  if ((code == 2) || (code == -2)) {
+    const bool perm = (code < 0);
+    if ((at == (T)0) && (!CPLX || (sat == (T)0)))
+      swap = (c == (T)1) && perm;
+    else {
+      const bool c1 = (c == (T)1);
+      const T cna = -at;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const T x_r = vx[j], y_r = vy[j];
+        if (!CPLX || (sat == (T)0)) {
+          T a = perm ? fma_(-x_r, at, y_r) : fma_(y_r, at, x_r);
+          T b = perm ? fma_(y_r, at, x_r) : fma_(-x_r, at, y_r);
+          if (!c1) { a = a * c; b = b * c; }
+          vx[j] = a;
+          vy[j] = b;
+          if constexpr (CPLX) {
+            const T x_i = vxi[j], y_i = vyi[j];
+            T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
+            T b2 = perm ? fma_(y_i, at, x_i) : fma_(-x_i, at, y_i);
+            if (!c1) { a2 = a2 * c; b2 = b2 * c; }
+            vxi[j] = a2;
+            vyi[j] = b2;
+          }
+        }
+        else if constexpr (CPLX) {
+          const T x_i = vxi[j], y_i = vyi[j];
+          T ar = fma_(y_r, at, fma_(-y_i, sat, x_r));
+          T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
+          T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
+          T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
+          if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+          if (perm) { vx[j] = br; vxi[j] = bi; vy[j] = ar; vyi[j] = ai; }
+          else { vx[j] = ar; vxi[j] = ai; vy[j] = br; vyi[j] = bi; }
+        }
+      }
+    }
+  }
+  if (swap) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      T t = vx[j]; vx[j] = vy[j]; vy[j] = t;
+      if constexpr (CPLX) { t = vxi[j]; vxi[j] = vyi[j]; vyi[j] = t; }
+    }
+  }
+}
+
+// Minimum waves per SIMD the register allocation must allow: two, which keeps
+// the double-buffered column pairs in registers, except for the widest
+// complex pairs, which would spill at 256 registers and run one wave per
+// SIMD instead.
+template <typename T, bool CPLX, int E> struct BsvdMinWaves {
+  static constexpr int value = (CPLX ? (E * (int)sizeof(T) >= 32) : (E * (int)sizeof(T) >= 128)) ? 1 : 2;
+};
+
 template <typename T, bool CPLX, int E, int W>
-__global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
+__global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_bsvd(const BatchArgs<T> a)
 {
   constexpr int B = 64;
-  __shared__ PairScal<T> s_scal[64];           // phase A -> B/C (n <= 128)
-  __shared__ PairDec<T> s_dec[64];             // phase B -> C
   __shared__ int s_b;
   __shared__ double s_max[2][W][2];            // per-wave (tG, tV) maxima, by step parity
   __shared__ unsigned long long s_cnt[2][W];   // per-wave transformation counts, by step parity
@@ -111,7 +242,6 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
     T *const Vi = CPLX ? a.Vi + a.offV[b] : nullptr;
     T *const eS = a.eS + a.offS[b];
     T *const fS = a.fS + a.offS[b];
-    const unsigned *const js = a.tables + a.tab_off[n];
     const unsigned stp = (unsigned)(n - 1);
     const int nE = (m + 63) >> 6;
 
@@ -176,80 +306,65 @@ __global__ __launch_bounds__(64 * W) void k_bsvd(const BatchArgs<T> a)
       for (sw = 0; sw < a.swp;) {
         unsigned long long swt = 0;
         for (unsigned st = 0; st < stp; ++st) {
-          const unsigned *const row = js + (size_t)st * n;
           const int npairs = n >> 1;
-          // ---- phase A: norms and cosines of this wave's pairs -> LDS; the
-          // next pair's columns are in flight while this one is reduced
-          {
-            T xr[E], yr[E], xi[E], yi[E];
-            if (wave < npairs)
-              load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)row[2 * wave], (int)row[2 * wave + 1], S.sG, lane, xr, yr, xi, yi);
-            for (int k = wave; k < npairs; k += W) {
-              T nxr[E], nyr[E], nxi[E], nyi[E];
-              const int kn = k + W;
-              if (kn < npairs)
-                load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)row[2 * kn], (int)row[2 * kn + 1], S.sG, lane, nxr, nyr, nxi, nyi);
-              const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, nullptr, nE);
-              if (lane == 0)
-                s_scal[k] = P;
-#pragma unroll
-              for (int j = 0; j < E; ++j) {
-                xr[j] = nxr[j];
-                yr[j] = nyr[j];
-                xi[j] = nxi[j];
-                yi[j] = nyi[j];
-              }
-            }
-          }
-          __syncthreads();
-          // ---- phase B: decisions and 2x2 EVDs of all pairs, one lane per pair
-          for (int k = threadIdx.x; k < npairs; k += 64 * W)
-            s_dec[k] = pair_decide<T, CPLX>(s_scal[k], S.tol, S.gst);
-          __syncthreads();
-          // ---- phase C: updates
+          // One wave per pair, the pair's whole step in registers: reduce,
+          // decide (every lane of the wave solves the same 2x2 problem), apply
+          // and store.  The next pair's G and V columns are loaded before this
+          // pair is reduced; the pairs are the round-robin pairs of pjs(2, n).
           T wG = (T)0, wV = (T)0;
           unsigned long long wcnt = 0;
           int werr = 0;
-          for (int k = wave; k < npairs; k += W) {
-            const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
-            const PairScal<T> P = s_scal[k];
-            const PairDec<T> D = s_dec[k];
-            T *const Vp = Vr + (long long)p * n, *const Vq = Vr + (long long)q * n;
-            T *const Vpi = CPLX ? Vi + (long long)p * n : nullptr, *const Vqi = CPLX ? Vi + (long long)q * n : nullptr;
-            if (S.sV) {
-              for (int i = lane; i < n; i += B) {
-                Vp[i] = ldexp_(Vp[i], S.sV);
-                Vq[i] = ldexp_(Vq[i], S.sV);
-                if constexpr (CPLX) {
-                  Vpi[i] = ldexp_(Vpi[i], S.sV);
-                  Vqi[i] = ldexp_(Vqi[i], S.sV);
+          if (wave < npairs) {
+            unsigned p, q;
+            rr_pair((unsigned)n, st, (unsigned)wave, p, q);
+            T xr[E], yr[E], xi[E], yi[E], vx[2], vy[2], vxi[2], vyi[2];
+            load_vpair<T, CPLX>(Vr, Vi, n, (int)p, (int)q, lane, vx, vy, vxi, vyi);
+            load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p, (int)q, lane, xr, yr, xi, yi);
+            for (int k = wave; k < npairs; k += W) {
+              // next pair of this wave (this pair again after the last one)
+              const int kn = (k + W < npairs) ? (k + W) : k;
+              unsigned pn, qn;
+              rr_pair((unsigned)n, st, (unsigned)kn, pn, qn);
+              T nxr[E], nyr[E], nxi[E], nyi[E], nvx[2], nvy[2], nvxi[2], nvyi[2];
+              load_vpair<T, CPLX>(Vr, Vi, n, (int)pn, (int)qn, lane, nvx, nvy, nvxi, nvyi);
+              load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)pn, (int)qn, lane, nxr, nyr, nxi, nyi);
+              ready_pair<T, CPLX, E>(m, nE, S.sG, lane, xr, yr, xi, yi);
+              const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, nullptr, nE);
+              const PairDec<T> D = pair_decide<T, CPLX>(P, S.tol, S.gst);
+              T eSp, fSp, eSq, fSq;
+              const PairOut<T, CPLX> o = pair_apply<T, CPLX, B, E>(0, xr, yr, xi, yi, nullptr, nullptr, nullptr, nullptr, P, D,
+                                                                   0, eSp, fSp, eSq, fSq, nullptr, lane, nE);
+              if (!o.err && ((D.code != 1) || S.sG))
+                store_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p, (int)q, lane, xr, yr, xi, yi);
+              if (S.sV || (!D.err && (D.code != 1) && (D.code != 3))) {
+                if (S.sV)
+                  scale_vpair<T>(S.sV, vx, vy, vxi, vyi);
+                if (!D.err)
+                  v_update<T, CPLX>(D, vx, vy, vxi, vyi);
+                store_vpair<T, CPLX>(Vr, Vi, n, (int)p, (int)q, lane, vx, vy, vxi, vyi);
+              }
+              if (o.err)
+                werr = o.err;
+              else {
+                if (lane == 0) {
+                  eS[p] = eSp; fS[p] = fSp;
+                  eS[q] = eSq; fS[q] = fSq;
                 }
+                wG = xmax(wG, o.tG);
+                wV = xmax(wV, o.tV);
+                wcnt += (unsigned long long)o.trans;
+              }
+              p = pn;
+              q = qn;
+#pragma unroll
+              for (int j = 0; j < E; ++j) {
+                xr[j] = nxr[j]; yr[j] = nyr[j]; xi[j] = nxi[j]; yi[j] = nyi[j];
+              }
+#pragma unroll
+              for (int j = 0; j < 2; ++j) {
+                vx[j] = nvx[j]; vy[j] = nvy[j]; vxi[j] = nvxi[j]; vyi[j] = nvyi[j];
               }
             }
-            T eSp, fSp, eSq, fSq;
-            PairOut<T, CPLX> o;
-            if ((D.code != 1) || S.sG || D.err) {
-              T xr[E], yr[E], xi[E], yi[E];
-              load_pair<T, CPLX, E>(Gr, Gi, m, nE, p, q, S.sG, lane, xr, yr, xi, yi);
-              o = pair_apply<T, CPLX, B, E>(n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane, nE);
-              if (!o.err)
-                store_pair<T, CPLX, E>(Gr, Gi, m, nE, p, q, lane, xr, yr, xi, yi);
-            }
-            else {
-              o = PairOut<T, CPLX>{1, 0, 0, (T)0, (T)0};
-              eSp = P.ex; fSp = P.fx; eSq = P.ey; fSq = P.fy;
-            }
-            if (o.err) {
-              werr = o.err;
-              continue;
-            }
-            if (lane == 0) {
-              eS[p] = eSp; fS[p] = fSp;
-              eS[q] = eSq; fS[q] = fSq;
-            }
-            wG = xmax(wG, o.tG);
-            wV = xmax(wV, o.tV);
-            wcnt += (unsigned long long)o.trans;
           }
           if (lane == 0) {
             s_max[par][wave][0] = (double)wG;

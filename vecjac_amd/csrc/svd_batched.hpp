@@ -11,8 +11,6 @@ template <typename T> struct BatchArgs {
   const long long *offG, *offV;   // element offsets of G_b (ld m_b) and V_b (ld n_b)
   const long long *offS;          // offsets of eS_b / fS_b
   const int *order;               // processing order (decreasing cost)
-  const unsigned *tables;         // concatenated ME tables
-  const long long *tab_off;       // tab_off[n]: offset of the table of width n
   T *Gr, *Gi, *Vr, *Vi, *eS, *fS; // data (Gi/Vi null for real data)
   long long *info;                // per matrix: sweeps, or a negative code
   unsigned long long *transforms; // per matrix: transformations (may be null)

@@ -148,17 +148,41 @@ template <typename T> __device__ __forceinline__ DD<T> row_sum(DD<T> v)
   return v;
 }
 
+// Broadcast of row values over the wave, kept in VGPRs (readlane would
+// make them scalar and crowd the SGPR file): k holds k_r in every lane of
+// row r; afterwards q_r holds k_r in every lane.
+template <typename T> __device__ __forceinline__ void bcast_rows4(const T k, T &q0, T &q1, T &q2, T &q3)
+{
+  T a = k, b = k;
+  swap_<true>(a, b);     // a: rows (k0, k1, k0, k1), b: rows (k2, k3, k2, k3)
+  q0 = a;
+  q1 = a;
+  swap_<false>(q0, q1);  // q0: k0 everywhere, q1: k1 everywhere
+  q2 = b;
+  q3 = b;
+  swap_<false>(q2, q3);
+}
+template <typename T> __device__ __forceinline__ void bcast_rows4(const DD<T> k, DD<T> &q0, DD<T> &q1, DD<T> &q2, DD<T> &q3)
+{
+  bcast_rows4(k.s, q0.s, q1.s, q2.s, q3.s);
+  bcast_rows4(k.c, q0.c, q1.c, q2.c, q3.c);
+}
+
 // Max of two values over the group.  Wave level: one swap32 scatters a to
-// the low half and b to the high half, the rows reduce, and lanes 0/16
-// (a) and 32/48 (b) are combined.
+// the low half and b to the high half, the rows reduce with DPP, rows 0/1
+// and 2/3 combine through one swap16 and a last swap32 spreads a and b.
 template <typename T, int B>
 __device__ __forceinline__ void red_max2(T &a, T &b, double *lds)
 {
   T lo = a, hi = b;
   swap_<true>(lo, hi);
   const T k = row_max(xmax(lo, hi));
-  a = xmax(readlane_(k, 0), readlane_(k, 16));
-  b = xmax(readlane_(k, 32), readlane_(k, 48));
+  T c = k, d = k;
+  swap_<false>(c, d);    // c: rows (k0, k0, k2, k2), d: rows (k1, k1, k3, k3)
+  T x = xmax(c, d), y = x;
+  swap_<true>(x, y);     // x: a everywhere, y: b everywhere
+  a = x;
+  b = y;
   if constexpr (B > 64) {
     constexpr int W = B / 64;
     const int lane = threadIdx.x & 63;
@@ -183,17 +207,14 @@ __device__ __forceinline__ void red_max2(T &a, T &b, double *lds)
 // Sum of four double-double values over the group.  Wave level: swap32
 // scatters (q0, q2) and (q1, q3) over the halves, swap16 scatters the two
 // results over the rows (row r then holds a partial sum of q_r), the rows
-// reduce with DPP and lanes 0/16/32/48 hold q0..q3.
+// reduce with DPP and the row totals are spread to every lane.
 template <typename A, int B>
 __device__ __forceinline__ void red_dd4(DD<A> &q0, DD<A> &q1, DD<A> &q2, DD<A> &q3, double *lds)
 {
   const DD<A> k0 = dd_swapadd<true>(q0, q2);
   const DD<A> k1 = dd_swapadd<true>(q1, q3);
   const DD<A> k = row_sum(dd_swapadd<false>(k0, k1));
-  q0 = DD<A>{readlane_(k.s, 0), readlane_(k.c, 0)};
-  q1 = DD<A>{readlane_(k.s, 16), readlane_(k.c, 16)};
-  q2 = DD<A>{readlane_(k.s, 32), readlane_(k.c, 32)};
-  q3 = DD<A>{readlane_(k.s, 48), readlane_(k.c, 48)};
+  bcast_rows4(k, q0, q1, q2, q3);
   if constexpr (B > 64) {
     constexpr int W = B / 64;
     const int lane = threadIdx.x & 63;
@@ -225,7 +246,11 @@ template <typename T, int B>
 __device__ __forceinline__ T red_max(T v, double *lds)
 {
   const T k = row_max(v);
-  v = xmax(xmax(readlane_(k, 0), readlane_(k, 16)), xmax(readlane_(k, 32), readlane_(k, 48)));
+  T c = k, d = k;
+  swap_<false>(c, d);
+  T x = xmax(c, d), y = x;
+  swap_<true>(x, y);
+  v = xmax(x, y);
   if constexpr (B > 64) {
     constexpr int W = B / 64;
     const int w = threadIdx.x >> 6;
@@ -241,6 +266,14 @@ __device__ __forceinline__ T red_max(T v, double *lds)
   }
   return v;
 }
+
+// Running maximum of |updated element| for updates with one product per
+// element (real rotations and Gram-Schmidt steps, complex rotations with a
+// real tangent): finite columns and finite parameters give finite values or
+// +-inf, never NaN, and there the IEEE maximum (one instruction) equals the
+// reference's ordered maximum.  Updates that sum two products can make
+// inf - inf and keep xmax.
+template <typename T> __device__ __forceinline__ T max1p(T acc, T v) { return __builtin_fmax(acc, v); }
 
 // ------------------------------------------------------------- (e,f) norms
 // Accumulator type of the norm and dot reductions: double-double for FP64
@@ -492,14 +525,14 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
         if (!c1) { a = a * c; b = b * c; }
         xr[j] = a;
         yr[j] = b;
-        tG = xmax(tG, xmax(vabs(a), vabs(b)));
+        tG = max1p(tG, max1p(vabs(a), vabs(b)));
         if constexpr (CPLX) {
           T a2 = perm ? fma_(-xi[j], at, yi[j]) : fma_(yi[j], at, xi[j]);
           T b2 = perm ? fma_(yi[j], at, xi[j]) : fma_(-xi[j], at, yi[j]);
           if (!c1) { a2 = a2 * c; b2 = b2 * c; }
           xi[j] = a2;
           yi[j] = b2;
-          tG = xmax(tG, xmax(vabs(a2), vabs(b2)));
+          tG = max1p(tG, max1p(vabs(a2), vabs(b2)));
         }
       }
     }
@@ -544,7 +577,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
           if (!c1) { a = a * c; b = b * c; }
           Vrp[i] = a;
           Vrq[i] = b;
-          tV = xmax(tV, xmax(vabs(a), vabs(b)));
+          tV = max1p(tV, max1p(vabs(a), vabs(b)));
           if constexpr (CPLX) {
             const T x_i = Vip[i], y_i = Viq[i];
             T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
@@ -552,7 +585,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
             if (!c1) { a2 = a2 * c; b2 = b2 * c; }
             Vip[i] = a2;
             Viq[i] = b2;
-            tV = xmax(tV, xmax(vabs(a2), vabs(b2)));
+            tV = max1p(tV, max1p(vabs(a2), vabs(b2)));
           }
         }
         else if constexpr (CPLX) {
@@ -601,7 +634,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
         }
         else {
           yr[j] = scalef_fin(fma_(tfr, xs, ys), ey);
-          tG = xmax(tG, vabs(yr[j]));
+          tG = max1p(tG, vabs(yr[j]));
         }
       }
       eSp = ex; fSp = fx; eSq = ey; fSq = fy;
@@ -629,7 +662,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
         else {
           xr[j] = yr[j];
           yr[j] = scalef_fin(fma_(tfr, ys, xs), ex);
-          tG = xmax(tG, vabs(yr[j]));
+          tG = max1p(tG, vabs(yr[j]));
         }
       }
       for (int i = lane; i < n; i += B) {
