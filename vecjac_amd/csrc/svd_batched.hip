@@ -207,6 +207,192 @@ __device__ __forceinline__ void v_update(const PairDec<T> &D, T (&vx)[2], T (&vy
   }
 }
 
+// Pairs a wave works on together (their reductions share the cross-lane
+// exchanges and their 2x2 decisions run side by side on different lanes),
+// bounded by the registers of the double-buffered column pairs: 2 for up to
+// 8 registers per column, else 1.
+template <typename T, bool CPLX, int E> struct GroupPairs {
+  static constexpr int rc = E * (CPLX ? 2 : 1) * (int)(sizeof(T) / 4); // registers per column
+  static constexpr int value = (rc <= 8) ? 2 : 1;
+};
+
+// Max of four values over the wave, every lane receiving all four totals.
+template <typename T> __device__ __forceinline__ void red_max4(T &a, T &b, T &c, T &d)
+{
+  T a1 = a, c1 = c, b1 = b, d1 = d;
+  swap_<true>(a1, c1);   // lanes < 32: a, lanes >= 32: c
+  swap_<true>(b1, d1);
+  T k0 = xmax(a1, c1), k1 = xmax(b1, d1);
+  swap_<false>(k0, k1);  // rows (a, b, c, d)
+  const T k = row_max(xmax(k0, k1));
+  bcast_rows4(k, a, b, c, d);
+}
+
+// Value of lane r of each quad, in every lane of the quad (DPP quad_perm).
+template <int R> __device__ __forceinline__ int qb_(int v)
+{
+  return __builtin_amdgcn_update_dpp(0, v, R | (R << 2) | (R << 4) | (R << 6), 0xF, 0xF, false);
+}
+template <int R> __device__ __forceinline__ double qb_(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = qb_<R>((int)b), hi = qb_<R>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int R> __device__ __forceinline__ float qb_(float v) { return __int_as_float(qb_<R>(__float_as_int(v))); }
+template <int R, typename T> __device__ __forceinline__ PairScal<T> qb_(const PairScal<T> &P)
+{
+  return PairScal<T>{qb_<R>(P.ex), qb_<R>(P.fx), qb_<R>(P.ey), qb_<R>(P.fy), qb_<R>(P.a21r), qb_<R>(P.a21i)};
+}
+template <int R, typename T> __device__ __forceinline__ PairDec<T> qb_(const PairDec<T> &D)
+{
+  return PairDec<T>{qb_<R>(D.code), qb_<R>(D.trans), qb_<R>(D.err), qb_<R>(D.c), qb_<R>(D.at), qb_<R>(D.sat)};
+}
+
+// One step of the strategy for the pairs of one wave (pairs wave, wave + W,
+// wave + 2W, ... of the round-robin table of width n), GP pairs at a time:
+// load (the next group is prefetched), reduce the GP pairs together, decide
+// with lane r of every quad working on pair r, broadcast each decision and
+// apply it in registers, store.  Accumulates this wave's per-lane max of the
+// updated G elements, its transformation count and its first error.
+template <typename T, bool CPLX, int E, int W, int GP>
+__device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n, const int nE,
+                                          const unsigned st, const int wave, const int lane, const Step<T> &S, T &wG,
+                                          unsigned long long &wcnt, int &werr)
+{
+  typedef typename Acc<T>::A A;
+  constexpr int B = 64;
+  const int npairs = n >> 1;
+  if (wave >= npairs)
+    return;
+  const int cnt = (npairs - wave + W - 1) / W;       // pairs of this wave
+  const int ngr = (cnt + GP - 1) / GP;
+  T xr[GP][E], yr[GP][E], xi[GP][E], yi[GP][E], vx[GP][2], vy[GP][2], vxi[GP][2], vyi[GP][2];
+  unsigned p[GP], q[GP];
+  // pair r of group g (a group past the end repeats the last pair)
+  auto pair_of = [&](const int g, const int r, unsigned &pp, unsigned &qq) {
+    int i = g * GP + r;
+    i = (i < cnt) ? i : (cnt - 1);
+    rr_pair((unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
+  };
+#pragma unroll
+  for (int r = 0; r < GP; ++r) {
+    pair_of(0, r, p[r], q[r]);
+    load_vpair<T, CPLX>(Vr, Vi, n, (int)p[r], (int)q[r], lane, vx[r], vy[r], vxi[r], vyi[r]);
+    load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p[r], (int)q[r], lane, xr[r], yr[r], xi[r], yi[r]);
+  }
+  for (int g = 0; g < ngr; ++g) {
+    // prefetch of the next group (this group again after the last one)
+    const int gn = (g + 1 < ngr) ? (g + 1) : g;
+    unsigned pn[GP], qn[GP];
+    T nxr[GP][E], nyr[GP][E], nxi[GP][E], nyi[GP][E], nvx[GP][2], nvy[GP][2], nvxi[GP][2], nvyi[GP][2];
+#pragma unroll
+    for (int r = 0; r < GP; ++r) {
+      pair_of(gn, r, pn[r], qn[r]);
+      load_vpair<T, CPLX>(Vr, Vi, n, (int)pn[r], (int)qn[r], lane, nvx[r], nvy[r], nvxi[r], nvyi[r]);
+      load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)pn[r], (int)qn[r], lane, nxr[r], nyr[r], nxi[r], nyi[r]);
+    }
+    // ---- reduce the GP pairs together
+    T mx[GP], my[GP];
+    int kx[GP], ky[GP];
+    DD<A> sx[GP], sy[GP], dr[GP], di[GP];
+#pragma unroll
+    for (int r = 0; r < GP; ++r) {
+      ready_pair<T, CPLX, E>(m, nE, S.sG, lane, xr[r], yr[r], xi[r], yi[r]);
+      pair_maxima<T, CPLX, E>(xr[r], yr[r], xi[r], yi[r], nE, mx[r], my[r]);
+    }
+    if constexpr (GP == 1)
+      red_max2<T, B>(mx[0], my[0], nullptr);
+    else {
+#pragma unroll
+      for (int r = 0; r < GP; r += 2)
+        red_max4(mx[r], my[r], mx[r + 1], my[r + 1]);
+    }
+#pragma unroll
+    for (int r = 0; r < GP; ++r) {
+      kx[r] = max_exp(mx[r]);
+      ky[r] = max_exp(my[r]);
+      pair_sums<T, CPLX, E>(xr[r], yr[r], xi[r], yi[r], nE, kx[r], ky[r], sx[r], sy[r], dr[r], di[r]);
+      red_dd4<A, B>(sx[r], sy[r], dr[r], di[r], nullptr);
+    }
+    // ---- lane r of every quad finishes and decides pair r
+    const int rl = lane & (GP - 1);
+    T lmx = mx[0], lmy = my[0];
+    int lkx = kx[0], lky = ky[0];
+    DD<A> lsx = sx[0], lsy = sy[0], ldr = dr[0], ldi = di[0];
+#pragma unroll
+    for (int r = 1; r < GP; ++r) {
+      const bool t = (rl == r);
+      lmx = t ? mx[r] : lmx;
+      lmy = t ? my[r] : lmy;
+      lkx = t ? kx[r] : lkx;
+      lky = t ? ky[r] : lky;
+      lsx = dd_sel(t, sx[r], lsx);
+      lsy = dd_sel(t, sy[r], lsy);
+      ldr = dd_sel(t, dr[r], ldr);
+      ldi = dd_sel(t, di[r], ldi);
+    }
+    const PairScal<T> Pl = pair_scal<T, CPLX>(lmx, lmy, lkx, lky, lsx, lsy, ldr, ldi);
+    const PairDec<T> Dl = pair_decide<T, CPLX>(Pl, S.tol, S.gst);
+    // ---- apply and store, pair by pair
+#pragma unroll
+    for (int r = 0; r < GP; ++r) {
+      if (g * GP + r >= cnt)
+        break;
+      PairScal<T> P;
+      PairDec<T> D;
+      if constexpr (GP == 1) {
+        P = Pl;
+        D = Dl;
+      }
+      else if (r == 0) { P = qb_<0>(Pl); D = qb_<0>(Dl); }
+      else if (r == 1) { P = qb_<1>(Pl); D = qb_<1>(Dl); }
+      else if (r == 2) { P = qb_<2>(Pl); D = qb_<2>(Dl); }
+      else { P = qb_<3>(Pl); D = qb_<3>(Dl); }
+      T eSp, fSp, eSq, fSq;
+      const PairOut<T, CPLX> o = pair_apply<T, CPLX, B, E, false>(0, xr[r], yr[r], xi[r], yi[r], nullptr, nullptr, nullptr,
+                                                                  nullptr, P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane, nE);
+      // an update that overflowed anywhere in the wave (the reference's
+      // max-|.| test after the update) leaves G unstored
+      int err = D.err;
+      if (!err && (__ballot(!isfinite_(o.tG)) != 0ull))
+        err = CPLX ? -630 : -580;
+      if (!err && ((D.code != 1) || S.sG))
+        store_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p[r], (int)q[r], lane, xr[r], yr[r], xi[r], yi[r]);
+      if (S.sV || (!D.err && (D.code != 1) && (D.code != 3))) {
+        if (S.sV)
+          scale_vpair<T>(S.sV, vx[r], vy[r], vxi[r], vyi[r]);
+        if (!D.err)
+          v_update<T, CPLX>(D, vx[r], vy[r], vxi[r], vyi[r]);
+        store_vpair<T, CPLX>(Vr, Vi, n, (int)p[r], (int)q[r], lane, vx[r], vy[r], vxi[r], vyi[r]);
+      }
+      if (err)
+        werr = err;
+      else {
+        if (lane == 0) {
+          eS[p[r]] = eSp; fS[p[r]] = fSp;
+          eS[q[r]] = eSq; fS[q[r]] = fSq;
+        }
+        wG = xmax(wG, o.tG);
+        wcnt += (unsigned long long)o.trans;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < GP; ++r) {
+      p[r] = pn[r];
+      q[r] = qn[r];
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        xr[r][j] = nxr[r][j]; yr[r][j] = nyr[r][j]; xi[r][j] = nxi[r][j]; yi[r][j] = nyi[r][j];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        vx[r][j] = nvx[r][j]; vy[r][j] = nvy[r][j]; vxi[r][j] = nvxi[r][j]; vyi[r][j] = nvyi[r][j];
+      }
+    }
+  }
+}
+
 // Minimum waves per SIMD the register allocation must allow: two, which keeps
 // the double-buffered column pairs in registers, except for the widest
 // complex pairs, which would spill at 256 registers and run one wave per
@@ -306,66 +492,23 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       for (sw = 0; sw < a.swp;) {
         unsigned long long swt = 0;
         for (unsigned st = 0; st < stp; ++st) {
-          const int npairs = n >> 1;
-          // One wave per pair, the pair's whole step in registers: reduce,
-          // decide (every lane of the wave solves the same 2x2 problem), apply
-          // and store.  The next pair's G and V columns are loaded before this
-          // pair is reduced; the pairs are the round-robin pairs of pjs(2, n).
           T wG = (T)0, wV = (T)0;
           unsigned long long wcnt = 0;
           int werr = 0;
-          if (wave < npairs) {
-            unsigned p, q;
-            rr_pair((unsigned)n, st, (unsigned)wave, p, q);
-            T xr[E], yr[E], xi[E], yi[E], vx[2], vy[2], vxi[2], vyi[2];
-            load_vpair<T, CPLX>(Vr, Vi, n, (int)p, (int)q, lane, vx, vy, vxi, vyi);
-            load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p, (int)q, lane, xr, yr, xi, yi);
-            for (int k = wave; k < npairs; k += W) {
-              // next pair of this wave (this pair again after the last one)
-              const int kn = (k + W < npairs) ? (k + W) : k;
-              unsigned pn, qn;
-              rr_pair((unsigned)n, st, (unsigned)kn, pn, qn);
-              T nxr[E], nyr[E], nxi[E], nyi[E], nvx[2], nvy[2], nvxi[2], nvyi[2];
-              load_vpair<T, CPLX>(Vr, Vi, n, (int)pn, (int)qn, lane, nvx, nvy, nvxi, nvyi);
-              load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)pn, (int)qn, lane, nxr, nyr, nxi, nyi);
-              ready_pair<T, CPLX, E>(m, nE, S.sG, lane, xr, yr, xi, yi);
-              const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, nullptr, nE);
-              const PairDec<T> D = pair_decide<T, CPLX>(P, S.tol, S.gst);
-              T eSp, fSp, eSq, fSq;
-              const PairOut<T, CPLX> o = pair_apply<T, CPLX, B, E>(0, xr, yr, xi, yi, nullptr, nullptr, nullptr, nullptr, P, D,
-                                                                   0, eSp, fSp, eSq, fSq, nullptr, lane, nE);
-              if (!o.err && ((D.code != 1) || S.sG))
-                store_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p, (int)q, lane, xr, yr, xi, yi);
-              if (S.sV || (!D.err && (D.code != 1) && (D.code != 3))) {
-                if (S.sV)
-                  scale_vpair<T>(S.sV, vx, vy, vxi, vyi);
-                if (!D.err)
-                  v_update<T, CPLX>(D, vx, vy, vxi, vyi);
-                store_vpair<T, CPLX>(Vr, Vi, n, (int)p, (int)q, lane, vx, vy, vxi, vyi);
-              }
-              if (o.err)
-                werr = o.err;
-              else {
-                if (lane == 0) {
-                  eS[p] = eSp; fS[p] = fSp;
-                  eS[q] = eSq; fS[q] = fSq;
-                }
-                wG = xmax(wG, o.tG);
-                wV = xmax(wV, o.tV);
-                wcnt += (unsigned long long)o.trans;
-              }
-              p = pn;
-              q = qn;
-#pragma unroll
-              for (int j = 0; j < E; ++j) {
-                xr[j] = nxr[j]; yr[j] = nyr[j]; xi[j] = nxi[j]; yi[j] = nyi[j];
-              }
-#pragma unroll
-              for (int j = 0; j < 2; ++j) {
-                vx[j] = nvx[j]; vy[j] = nvy[j]; vxi[j] = nvxi[j]; vyi[j] = nvyi[j];
-              }
-            }
-          }
+          // register footprint by this matrix's row count (chunks of 64 rows)
+          if ((E >= 2) && (nE <= 2))
+            wave_step<T, CPLX, (E >= 2 ? 2 : E), W, GroupPairs<T, CPLX, (E >= 2 ? 2 : E)>::value>(
+                Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
+          else if ((E >= 4) && (nE <= 4))
+            wave_step<T, CPLX, (E >= 4 ? 4 : E), W, GroupPairs<T, CPLX, (E >= 4 ? 4 : E)>::value>(
+                Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
+          else if ((E >= 8) && (nE <= 8))
+            wave_step<T, CPLX, (E >= 8 ? 8 : E), W, GroupPairs<T, CPLX, (E >= 8 ? 8 : E)>::value>(
+                Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
+          else
+            wave_step<T, CPLX, E, W, GroupPairs<T, CPLX, E>::value>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S,
+                                                                   wG, wcnt, werr);
+          wG = red_max<T, 64>(wG, nullptr);
           if (lane == 0) {
             s_max[par][wave][0] = (double)wG;
             s_max[par][wave][1] = (double)wV;

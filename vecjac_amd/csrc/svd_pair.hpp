@@ -111,6 +111,11 @@ template <typename T> __device__ __forceinline__ DD<T> dd_add(DD<T> a, DD<T> b)
   return r;
 }
 
+template <typename T> __device__ __forceinline__ DD<T> dd_sel(bool c, DD<T> a, DD<T> b)
+{
+  return DD<T>{c ? a.s : b.s, c ? a.c : b.c};
+}
+
 // s := s + p exactly kept in (s, c)
 template <typename T> __device__ __forceinline__ void dd_acc(DD<T> &a, T p, T e)
 {
@@ -338,14 +343,16 @@ template <typename T, bool CPLX> struct PairOut {
   T tG, tV;          // max |updated G / V element| of this pair
 };
 
-template <typename T, bool CPLX, int B, int E>
-__device__ __forceinline__ PairScal<T>
-pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E], double *lds, const int nE = E)
+// Per-lane pieces of pair_reduce, so that a wave can reduce several pairs
+// together.  pair_maxima: partial max |x|, max |y| over this lane's rows (G
+// is finite whenever a pair is reduced, checked at set-up and after every
+// update, so the IEEE maximum equals the ordered maximum here).
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ void pair_maxima(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E],
+                                            const int nE, T &mx, T &my)
 {
-  typedef typename Acc<T>::A A;
-  // G is finite whenever a pair is reduced (checked at set-up and after
-  // every update), so the IEEE maximum equals the ordered maximum here
-  T mx = (T)0, my = (T)0;
+  mx = (T)0;
+  my = (T)0;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     if (j < nE) {
@@ -357,10 +364,21 @@ pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[
       }
     }
   }
-  red_max2<T, B>(mx, my, lds);
-  const int kx = (mx > (T)0) ? (frexp_exp(mx) - 1) : 0;
-  const int ky = (my > (T)0) ? (frexp_exp(my) - 1) : 0;
-  DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
+}
+
+// Scaling exponent of a column from its max |.| (0 for a zero column).
+template <typename T> __device__ __forceinline__ int max_exp(const T mx) { return (mx > (T)0) ? (frexp_exp(mx) - 1) : 0; }
+
+// pair_sums: partial double-double sums of |x 2^-kx|^2, |y 2^-ky|^2 and
+// x^H y 2^-(kx+ky) (real part dr, imaginary part di) over this lane's rows.
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ void pair_sums(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E],
+                                          const int nE, const int kx, const int ky, DD<typename Acc<T>::A> &sx,
+                                          DD<typename Acc<T>::A> &sy, DD<typename Acc<T>::A> &dr,
+                                          DD<typename Acc<T>::A> &di)
+{
+  typedef typename Acc<T>::A A;
+  sx = sy = dr = di = DD<A>{0, 0};
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     if (j >= nE)
@@ -402,7 +420,15 @@ pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[
       }
     }
   }
-  red_dd4<A, B>(sx, sy, dr, di, lds);
+}
+
+// pair_scal: the (e,f) norms and the cosine from the group totals.
+template <typename T, bool CPLX>
+__device__ __forceinline__ PairScal<T> pair_scal(const T mx, const T my, const int kx, const int ky,
+                                                 const DD<typename Acc<T>::A> sx, const DD<typename Acc<T>::A> sy,
+                                                 const DD<typename Acc<T>::A> dr, const DD<typename Acc<T>::A> di)
+{
+  typedef typename Acc<T>::A A;
   const double rx = (mx > (T)0) ? dd_sqrt(DD<double>{(double)sx.s, (double)sx.c}) : 0.0;
   const double ry = (my > (T)0) ? dd_sqrt(DD<double>{(double)sy.s, (double)sy.c}) : 0.0;
   PairScal<T> o;
@@ -422,6 +448,21 @@ pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[
       o.a21i = ldexp_((T)(-((A)di.s + (A)di.c)), shift) / de;
   }
   return o;
+}
+
+template <typename T, bool CPLX, int B, int E>
+__device__ __forceinline__ PairScal<T>
+pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E], double *lds, const int nE = E)
+{
+  typedef typename Acc<T>::A A;
+  T mx, my;
+  pair_maxima<T, CPLX, E>(xr, yr, xi, yi, nE, mx, my);
+  red_max2<T, B>(mx, my, lds);
+  const int kx = max_exp(mx), ky = max_exp(my);
+  DD<A> sx, sy, dr, di;
+  pair_sums<T, CPLX, E>(xr, yr, xi, yi, nE, kx, ky, sx, sy, dr, di);
+  red_dd4<A, B>(sx, sy, dr, di, lds);
+  return pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
 }
 
 // Convergence test, Gram prescaling, 2x2 EVD and action code of one pair
@@ -484,8 +525,9 @@ __device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T 
 // columns streamed.  Writes the pair's (e,f) as the reference leaves them
 // (swapped for codes -1 and -3; the norms of updated columns are recomputed
 // at their next step).  All lanes of the group call this with the same
-// decision.
-template <typename T, bool CPLX, int B, int E>
+// decision.  With RED = false the update maximum tG is left per lane (not
+// reduced over the group, not checked for finiteness): the caller folds it.
+template <typename T, bool CPLX, int B, int E, bool RED = true>
 __device__ __forceinline__ PairOut<T, CPLX>
 pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
            T *__restrict__ Vrp, T *__restrict__ Vrq, T *__restrict__ Vip, T *__restrict__ Viq,
@@ -556,7 +598,8 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
         }
       }
     }
-    tG = red_max<T, B>(tG, lds);
+    if constexpr (RED)
+      tG = red_max<T, B>(tG, lds);
     // V pair, streamed: same rotation with length +-n
     if ((at == (T)0) && (!CPLX || (sat == (T)0))) {
       if ((c == (T)1) && perm) {
@@ -605,7 +648,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
       else
         tV = (T)0;
     }
-    if (!isfinite_(tG)) { out.err = CPLX ? -630 : -580; return out; }
+    if (RED && !isfinite_(tG)) { out.err = CPLX ? -630 : -580; return out; }
     if (!isfinite_(tV)) { out.err = CPLX ? -638 : -588; return out; }
     eSp = ex; fSp = fx; eSq = ey; fSq = fy;
   }
@@ -671,8 +714,9 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
       }
       eSp = ey; fSp = fy; eSq = ex; fSq = fx;
     }
-    tG = red_max<T, B>(tG, lds);
-    if (!isfinite_(tG)) { out.err = CPLX ? -630 : -580; return out; }
+    if constexpr (RED)
+      tG = red_max<T, B>(tG, lds);
+    if (RED && !isfinite_(tG)) { out.err = CPLX ? -630 : -580; return out; }
   }
   else if (code == -1) {
 #pragma unroll
