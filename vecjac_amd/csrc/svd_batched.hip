@@ -208,12 +208,15 @@ __device__ __forceinline__ void v_update(const PairDec<T> &D, T (&vx)[2], T (&vy
 }
 
 // Pairs a wave works on together (their reductions share the cross-lane
-// exchanges and their 2x2 decisions run side by side on different lanes),
-// bounded by the registers of the double-buffered column pairs: 2 for up to
-// 8 registers per column, else 1.
-template <typename T, bool CPLX, int E> struct GroupPairs {
-  static constexpr int rc = E * (CPLX ? 2 : 1) * (int)(sizeof(T) / 4); // registers per column
-  static constexpr int value = (rc <= 8) ? 2 : 1;
+// exchanges and their 2x2 decisions run side by side on different lanes)
+// and whether the next group is prefetched into a second register set,
+// bounded by the registers per column (rc): two prefetched pairs up to 8,
+// two pairs without prefetch up to 16 (and E <= 8), else one prefetched
+// pair.
+template <typename T, bool CPLX, int E> struct StepShape {
+  static constexpr int rc = E * (CPLX ? 2 : 1) * (int)(sizeof(T) / 4);
+  static constexpr int GP = ((rc <= 16) && (E <= 8)) ? 2 : 1;
+  static constexpr bool PF = (rc <= 8) || (GP == 1);
 };
 
 // Max of four values over the wave, every lane receiving all four totals.
@@ -255,7 +258,7 @@ template <int R, typename T> __device__ __forceinline__ PairDec<T> qb_(const Pai
 // with lane r of every quad working on pair r, broadcast each decision and
 // apply it in registers, store.  Accumulates this wave's per-lane max of the
 // updated G elements, its transformation count and its first error.
-template <typename T, bool CPLX, int E, int W, int GP>
+template <typename T, bool CPLX, int E, int W, int GP, bool PF>
 __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n, const int nE,
                                           const unsigned st, const int wave, const int lane, const Step<T> &S, T &wG,
                                           unsigned long long &wcnt, int &werr)
@@ -275,22 +278,32 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
     i = (i < cnt) ? i : (cnt - 1);
     rr_pair((unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
   };
+  if constexpr (PF) {
 #pragma unroll
-  for (int r = 0; r < GP; ++r) {
-    pair_of(0, r, p[r], q[r]);
-    load_vpair<T, CPLX>(Vr, Vi, n, (int)p[r], (int)q[r], lane, vx[r], vy[r], vxi[r], vyi[r]);
-    load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p[r], (int)q[r], lane, xr[r], yr[r], xi[r], yi[r]);
+    for (int r = 0; r < GP; ++r) {
+      pair_of(0, r, p[r], q[r]);
+      load_vpair<T, CPLX>(Vr, Vi, n, (int)p[r], (int)q[r], lane, vx[r], vy[r], vxi[r], vyi[r]);
+      load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p[r], (int)q[r], lane, xr[r], yr[r], xi[r], yi[r]);
+    }
   }
   for (int g = 0; g < ngr; ++g) {
-    // prefetch of the next group (this group again after the last one)
-    const int gn = (g + 1 < ngr) ? (g + 1) : g;
+    // prefetch of the next group (this group again after the last one), or
+    // the loads of this group
+    const int gn = PF ? ((g + 1 < ngr) ? (g + 1) : g) : g;
     unsigned pn[GP], qn[GP];
     T nxr[GP][E], nyr[GP][E], nxi[GP][E], nyi[GP][E], nvx[GP][2], nvy[GP][2], nvxi[GP][2], nvyi[GP][2];
 #pragma unroll
     for (int r = 0; r < GP; ++r) {
-      pair_of(gn, r, pn[r], qn[r]);
-      load_vpair<T, CPLX>(Vr, Vi, n, (int)pn[r], (int)qn[r], lane, nvx[r], nvy[r], nvxi[r], nvyi[r]);
-      load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)pn[r], (int)qn[r], lane, nxr[r], nyr[r], nxi[r], nyi[r]);
+      if constexpr (PF) {
+        pair_of(gn, r, pn[r], qn[r]);
+        load_vpair<T, CPLX>(Vr, Vi, n, (int)pn[r], (int)qn[r], lane, nvx[r], nvy[r], nvxi[r], nvyi[r]);
+        load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)pn[r], (int)qn[r], lane, nxr[r], nyr[r], nxi[r], nyi[r]);
+      }
+      else {
+        pair_of(g, r, p[r], q[r]);
+        load_vpair<T, CPLX>(Vr, Vi, n, (int)p[r], (int)q[r], lane, vx[r], vy[r], vxi[r], vyi[r]);
+        load_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p[r], (int)q[r], lane, xr[r], yr[r], xi[r], yi[r]);
+      }
     }
     // ---- reduce the GP pairs together
     T mx[GP], my[GP];
@@ -377,6 +390,8 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
         wcnt += (unsigned long long)o.trans;
       }
     }
+    if constexpr (!PF)
+      continue;
 #pragma unroll
     for (int r = 0; r < GP; ++r) {
       p[r] = pn[r];
@@ -497,16 +512,16 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           int werr = 0;
           // register footprint by this matrix's row count (chunks of 64 rows)
           if ((E >= 2) && (nE <= 2))
-            wave_step<T, CPLX, (E >= 2 ? 2 : E), W, GroupPairs<T, CPLX, (E >= 2 ? 2 : E)>::value>(
+            wave_step<T, CPLX, (E >= 2 ? 2 : E), W, StepShape<T, CPLX, (E >= 2 ? 2 : E)>::GP, StepShape<T, CPLX, (E >= 2 ? 2 : E)>::PF>(
                 Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
           else if ((E >= 4) && (nE <= 4))
-            wave_step<T, CPLX, (E >= 4 ? 4 : E), W, GroupPairs<T, CPLX, (E >= 4 ? 4 : E)>::value>(
+            wave_step<T, CPLX, (E >= 4 ? 4 : E), W, StepShape<T, CPLX, (E >= 4 ? 4 : E)>::GP, StepShape<T, CPLX, (E >= 4 ? 4 : E)>::PF>(
                 Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
           else if ((E >= 8) && (nE <= 8))
-            wave_step<T, CPLX, (E >= 8 ? 8 : E), W, GroupPairs<T, CPLX, (E >= 8 ? 8 : E)>::value>(
+            wave_step<T, CPLX, (E >= 8 ? 8 : E), W, StepShape<T, CPLX, (E >= 8 ? 8 : E)>::GP, StepShape<T, CPLX, (E >= 8 ? 8 : E)>::PF>(
                 Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
           else
-            wave_step<T, CPLX, E, W, GroupPairs<T, CPLX, E>::value>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S,
+            wave_step<T, CPLX, E, W, StepShape<T, CPLX, E>::GP, StepShape<T, CPLX, E>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S,
                                                                    wG, wcnt, werr);
           wG = red_max<T, 64>(wG, nullptr);
           if (lane == 0) {
