@@ -43,6 +43,21 @@ def _host_has_avx512() -> bool:
         return False
 
 
+def _profiled_traffic(name: str):
+    """HBM bytes per launch of a kernel on its default workload, as measured
+    by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, each scaled by the
+    same counter on a copy of known size) and recorded in profiles/<name>.md
+    on a line `traffic_bytes_per_launch: <bytes>`.  None when absent."""
+    f = ROOT / "profiles" / f"{name}.md"
+    try:
+        for line in f.read_text().splitlines():
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].strip().split()[0]), f"profiles/{name}.md"
+    except (OSError, ValueError, IndexError):
+        pass
+    return None, None
+
+
 # --------------------------------------------------------------------------- 2x2 EVD
 def bench_bjac2(args, kind: str = "d"):
     import torch
@@ -98,6 +113,11 @@ def bench_bjac2(args, kind: str = "d"):
                      "kernel": f"k_{'c' if cplx else 'r'}bjac2", "kernel_avg_ms": k_avg_s * 1e3,
                      "alg_bytes_per_launch": traffic_alg},
     }
+    if int(args.lg) == 26:
+        t, src = _profiled_traffic(f"k_{'c' if cplx else 'r'}bjac2_{kind}_2e26")
+        out["roofline"]["traffic"] = t
+        if src:
+            out["roofline"]["traffic_source"] = src
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_bjac2(kind, ins_np)
     return out
@@ -201,6 +221,11 @@ def bench_bsvd(args, kind: str = "d"):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bsvd",
                      "kernel_avg_ms": r["kernel_ms"], "alg_bytes_per_launch": alg},
     }
+    if nb == 16384 and args.waves == 0:
+        t, src = _profiled_traffic(f"k_bsvd_{'fp64' if kind == 'd' else 'fp32'}_16384")
+        out["roofline"]["traffic"] = t
+        if src:
+            out["roofline"]["traffic_source"] = src
     # sigma accuracy on a sample, against the oracle on the same matrices
     if not args.no_check:
         out["sigma_check"] = _bsvd_sigma_check(m, npad, offG, offS, G64, r, kind)

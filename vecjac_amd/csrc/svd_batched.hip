@@ -58,14 +58,16 @@ __device__ __forceinline__ void ready_pair(const int m, const int nE, const int 
   for (int j = 0; j < E; ++j) {
     if (j >= nE)
       break;
-    const bool in = (lane + j * 64) < m;
-    xr[j] = in ? xr[j] : (T)0;
-    yr[j] = in ? yr[j] : (T)0;
-    if constexpr (CPLX) {
-      xi[j] = in ? xi[j] : (T)0;
-      yi[j] = in ? yi[j] : (T)0;
+    if ((j == nE - 1) && (m & 63)) {  // only the last chunk can be partial
+      const bool in = (lane + j * 64) < m;
+      xr[j] = in ? xr[j] : (T)0;
+      yr[j] = in ? yr[j] : (T)0;
+      if constexpr (CPLX) {
+        xi[j] = in ? xi[j] : (T)0;
+        yi[j] = in ? yi[j] : (T)0;
+      }
     }
-    else
+    if constexpr (!CPLX)
       xi[j] = yi[j] = (T)0;
     if (sG) {
       xr[j] = ldexp_(xr[j], sG);
@@ -231,25 +233,17 @@ template <typename T> __device__ __forceinline__ void red_max4(T &a, T &b, T &c,
   bcast_rows4(k, a, b, c, d);
 }
 
-// Value of lane r of each quad, in every lane of the quad (DPP quad_perm).
-template <int R> __device__ __forceinline__ int qb_(int v)
+// Pair r's scalars and decision from lane r, as wave-uniform values (so
+// that the update loops branch once on them instead of per element).
+template <typename T> __device__ __forceinline__ PairScal<T> lane_scal(const PairScal<T> &P, const int r)
 {
-  return __builtin_amdgcn_update_dpp(0, v, R | (R << 2) | (R << 4) | (R << 6), 0xF, 0xF, false);
+  return PairScal<T>{readlane_(P.ex, r), readlane_(P.fx, r), readlane_(P.ey, r), readlane_(P.fy, r),
+                     readlane_(P.a21r, r), readlane_(P.a21i, r)};
 }
-template <int R> __device__ __forceinline__ double qb_(double v)
+template <typename T> __device__ __forceinline__ PairDec<T> lane_dec(const PairDec<T> &D, const int r)
 {
-  const long long b = __double_as_longlong(v);
-  const int lo = qb_<R>((int)b), hi = qb_<R>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-template <int R> __device__ __forceinline__ float qb_(float v) { return __int_as_float(qb_<R>(__float_as_int(v))); }
-template <int R, typename T> __device__ __forceinline__ PairScal<T> qb_(const PairScal<T> &P)
-{
-  return PairScal<T>{qb_<R>(P.ex), qb_<R>(P.fx), qb_<R>(P.ey), qb_<R>(P.fy), qb_<R>(P.a21r), qb_<R>(P.a21i)};
-}
-template <int R, typename T> __device__ __forceinline__ PairDec<T> qb_(const PairDec<T> &D)
-{
-  return PairDec<T>{qb_<R>(D.code), qb_<R>(D.trans), qb_<R>(D.err), qb_<R>(D.c), qb_<R>(D.at), qb_<R>(D.sat)};
+  return PairDec<T>{__builtin_amdgcn_readlane(D.code, r), __builtin_amdgcn_readlane(D.trans, r),
+                    __builtin_amdgcn_readlane(D.err, r), readlane_(D.c, r), readlane_(D.at, r), readlane_(D.sat, r)};
 }
 
 // One step of the strategy for the pairs of one wave (pairs wave, wave + W,
@@ -328,7 +322,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
       pair_sums<T, CPLX, E>(xr[r], yr[r], xi[r], yi[r], nE, kx[r], ky[r], sx[r], sy[r], dr[r], di[r]);
       red_dd4<A, B>(sx[r], sy[r], dr[r], di[r], nullptr);
     }
-    // ---- lane r of every quad finishes and decides pair r
+    // ---- lane r (mod GP) finishes and decides pair r
     const int rl = lane & (GP - 1);
     T lmx = mx[0], lmy = my[0];
     int lkx = kx[0], lky = ky[0];
@@ -352,16 +346,8 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
     for (int r = 0; r < GP; ++r) {
       if (g * GP + r >= cnt)
         break;
-      PairScal<T> P;
-      PairDec<T> D;
-      if constexpr (GP == 1) {
-        P = Pl;
-        D = Dl;
-      }
-      else if (r == 0) { P = qb_<0>(Pl); D = qb_<0>(Dl); }
-      else if (r == 1) { P = qb_<1>(Pl); D = qb_<1>(Dl); }
-      else if (r == 2) { P = qb_<2>(Pl); D = qb_<2>(Dl); }
-      else { P = qb_<3>(Pl); D = qb_<3>(Dl); }
+      const PairScal<T> P = lane_scal(Pl, r);
+      const PairDec<T> D = lane_dec(Dl, r);
       T eSp, fSp, eSq, fSq;
       const PairOut<T, CPLX> o = pair_apply<T, CPLX, B, E, false>(0, xr[r], yr[r], xi[r], yi[r], nullptr, nullptr, nullptr,
                                                                   nullptr, P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane, nE);

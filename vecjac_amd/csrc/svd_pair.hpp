@@ -521,6 +521,37 @@ __device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T 
   return d;
 }
 
+// Real rotation of the register pair (djrot_ with a real tangent; for
+// complex data, a rotation with sat == 0 acts on the real and imaginary
+// parts alike): (x, y) := (x + t y, y - t x) c, the two results exchanged
+// when PERM, the scaling omitted when C1 (c == 1).  Returns the lane's max
+// |updated element|.
+template <typename T, bool CPLX, int E, bool PERM, bool C1>
+__device__ __forceinline__ T rot_real(T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E], const T c, const T at, const int nE)
+{
+  T tG = (T)0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
+    const T u = fma_(yr[j], at, xr[j]), v = fma_(-xr[j], at, yr[j]);
+    T a = PERM ? v : u, b = PERM ? u : v;
+    if constexpr (!C1) { a = a * c; b = b * c; }
+    xr[j] = a;
+    yr[j] = b;
+    tG = max1p(tG, max1p(vabs(a), vabs(b)));
+    if constexpr (CPLX) {
+      const T u2 = fma_(yi[j], at, xi[j]), v2 = fma_(-xi[j], at, yi[j]);
+      T a2 = PERM ? v2 : u2, b2 = PERM ? u2 : v2;
+      if constexpr (!C1) { a2 = a2 * c; b2 = b2 * c; }
+      xi[j] = a2;
+      yi[j] = b2;
+      tG = max1p(tG, max1p(vabs(a2), vabs(b2)));
+    }
+  }
+  return tG;
+}
+
 // In-place update of the pair (dvjsvd.c:460-574): x,y in registers, the V
 // columns streamed.  Writes the pair's (e,f) as the reference leaves them
 // (swapped for codes -1 and -3; the norms of updated columns are recomputed
@@ -557,26 +588,14 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
       }
     }
     else if (!CPLX || (sat == (T)0)) {
+      // the permutation and the unit cosine select the loop, not the element
       const bool c1 = (c == (T)1);
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        if (j >= nE)
-          break;
-        T a = perm ? fma_(-xr[j], at, yr[j]) : fma_(yr[j], at, xr[j]);
-        T b = perm ? fma_(yr[j], at, xr[j]) : fma_(-xr[j], at, yr[j]);
-        if (!c1) { a = a * c; b = b * c; }
-        xr[j] = a;
-        yr[j] = b;
-        tG = max1p(tG, max1p(vabs(a), vabs(b)));
-        if constexpr (CPLX) {
-          T a2 = perm ? fma_(-xi[j], at, yi[j]) : fma_(yi[j], at, xi[j]);
-          T b2 = perm ? fma_(yi[j], at, xi[j]) : fma_(-xi[j], at, yi[j]);
-          if (!c1) { a2 = a2 * c; b2 = b2 * c; }
-          xi[j] = a2;
-          yi[j] = b2;
-          tG = max1p(tG, max1p(vabs(a2), vabs(b2)));
-        }
-      }
+      if (perm)
+        tG = c1 ? rot_real<T, CPLX, E, true, true>(xr, yr, xi, yi, c, at, nE)
+                : rot_real<T, CPLX, E, true, false>(xr, yr, xi, yi, c, at, nE);
+      else
+        tG = c1 ? rot_real<T, CPLX, E, false, true>(xr, yr, xi, yi, c, at, nE)
+                : rot_real<T, CPLX, E, false, false>(xr, yr, xi, yi, c, at, nE);
     }
     else {
       if constexpr (CPLX) {
