@@ -180,6 +180,10 @@ def bench_bsvd(args, kind: str = "d"):
         info = torch.empty(nb, dtype=torch.int64, device="cuda")
         tr = torch.empty(nb, dtype=torch.int64, device="cuda")
         stream = torch.cuda.current_stream()
+        tune = None
+        if args.waves == 0 and not args.no_tune:
+            # plan-time autotuner (waves per workgroup), outside the timed region
+            tune = plan.autotune(G0, V, eS, fS, info)
         for _ in range(warmup):
             G.copy_(G0)
             plan.run(G, V, eS, fS, info, tr)
@@ -195,7 +199,7 @@ def bench_bsvd(args, kind: str = "d"):
         wall = time.perf_counter() - t0
         kms = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)]
         res = dict(plan=plan, wall=wall, kernel_ms=float(np.mean(kms)), info=info.cpu().numpy(),
-                   tr=tr.cpu().numpy(), G=G, eS=eS.cpu().numpy(), fS=fS.cpu().numpy(), waves=plan.waves)
+                   tr=tr.cpu().numpy(), G=G, eS=eS.cpu().numpy(), fS=fS.cpu().numpy(), waves=plan.waves, tune=tune)
         return res
 
     w = 8 if kind == "d" else 4
@@ -215,7 +219,7 @@ def bench_bsvd(args, kind: str = "d"):
         "config": {"workload": f"batched {'FP64' if kind == 'd' else 'FP32'} one-sided Jacobi SVD of {nb} matrices, "
                                "n ~ U{8..128} (odd n padded with a zero column), m ~ U{n..4n}, N(0,1) entries, "
                                "one persistent launch",
-                   "batch": nb, "waves_per_workgroup": r["waves"],
+                   "batch": nb, "waves_per_workgroup": r["waves"], "autotune_ms": r["tune"],
                    "sweeps_mean": float(np.mean(info)), "sweeps_max": int(np.max(info))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bsvd",
@@ -406,13 +410,67 @@ def bench_plain(args, kind: str):
     steps_run = (sweeps + 1) * stp
     alg = steps_run * w * m * n + R * w * (2 * m + 4 * n)
     achieved = alg / wall / 1e9
-    return {"metric": "SVDs/s (plain, one matrix)", "value": 1.0 / wall, "unit": "SVDs/s",
-            "ms_per_step": wall * 1e3, "dtype": "f64" if kind == "d" else "c128",
-            "config": {"workload": desc, "sweeps": int(sweeps), "launched_sweeps": int(hist[1]),
-                       "transformations": R},
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step (all launches)",
-                         "kernel_avg_ms": wall / max(1, int(hist[1]) * stp) * 1e3, "alg_bytes_per_launch": alg / max(1, steps_run)}}
+    out = {"metric": "SVDs/s (plain, one matrix)", "value": 1.0 / wall, "unit": "SVDs/s",
+           "ms_per_step": wall * 1e3, "dtype": "f64" if kind == "d" else "c128",
+           "config": {"workload": desc, "sweeps": int(sweeps), "launched_sweeps": int(hist[1]),
+                      "transformations": R},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step (all launches)",
+                        "kernel_avg_ms": wall / max(1, int(hist[1]) * stp) * 1e3,
+                        "alg_bytes_per_launch": alg / max(1, steps_run)}}
+    sg = np.sort(S.sigma(eS.cpu().numpy(), fS.cpu().numpy()))[::-1]
+    if kind == "z":
+        # against the nominal sigma of the construction: A is formed in FP64, so
+        # sigma_i moves by about eps |A| = eps; compared where that is far below 8 n eps sigma_i
+        sel = sig >= 1e-4
+        out["sigma_check"] = {"vs": "nominal sigma of A = Q1 diag(sigma) Q2^H, sigma_i >= 1e-4",
+                              "max_rel_err_over_8n_eps": float(np.max(np.abs(sg[sel] - sig[sel]) / sig[sel]) /
+                                                               (8 * n * np.finfo(np.float64).eps))}
+    if kind == "d" and not args.no_cpu:
+        out["cpu_baseline"], s_ref = _plain_cpu(A, m, n, js, stp)
+        if s_ref is not None:
+            s_ref = np.sort(s_ref)[::-1]
+            out["sigma_check"] = {"vs": out["cpu_baseline"]["kind"] + " dvjsvd_ on the host, same input and table",
+                                  "max_rel_err_over_8n_eps": float(np.max(np.abs(sg - s_ref) / s_ref) /
+                                                                   (8 * n * np.finfo(np.float64).eps))}
+    return out
+
+
+def _plain_cpu(A, m, n, js, stp):
+    """The reference's dvjsvd_ (its own sources, AVX-512, OpenMP over the host
+    threads) on the same matrix and table; the oracle port (serial) when the
+    reference build cannot run here.  Returns (baseline, sigma)."""
+    import ctypes as C
+    from oracle import vjo
+    from vecjac_amd import svd as S
+    threads = _cpu_threads()
+    G = vjo.aligned((n, m), np.float64)
+    G[...] = A.reshape(n, m)
+    jsa = vjo.aligned(js.shape, np.uint32)
+    jsa[...] = js
+    V = vjo.aligned((n, n), np.float64)
+    eS = vjo.aligned(n, np.float64)
+    fS = vjo.aligned(n, np.float64)
+    if vjo.ref() is not None and _host_has_avx512() and m % 8 == 0 and n % 16 == 0:
+        R = vjo.ref()
+        C.CDLL("libgomp.so.1").omp_set_num_threads(threads)
+        work = vjo.aligned(8 * n, np.float64)
+        iw = vjo.aligned(max(1, n // 8), np.uint32)
+        mm, nn = C.c_uint64(m), C.c_uint64(n)
+        t0 = time.perf_counter()
+        rc = R.dvjsvd_(C.byref(mm), C.byref(nn), G.ctypes.data, C.byref(mm), V.ctypes.data, C.byref(nn),
+                       eS.ctypes.data, fS.ctypes.data, jsa.ctypes.data, C.byref(C.c_uint(stp)), C.byref(C.c_uint(999)),
+                       work.ctypes.data, iw.ctypes.data)
+        t = time.perf_counter() - t0
+        kind, cores = "reference", threads
+    else:
+        t0 = time.perf_counter()
+        rc, eS, fS, *_ = vjo.vjsvd_core("d", G, js=js, stp=stp, swp=999)
+        t = time.perf_counter() - t0
+        kind, cores = "port", 1
+    base = {"value": 1.0 / t, "unit": "SVDs/s", "cores": cores, "kind": kind,
+            "sample": f"the whole {m}x{n} SVD, one call ({t:.1f} s, {int(rc)} sweeps)"}
+    return base, (S.sigma(eS, fS) if rc >= 0 else None)
 
 
 def bench_copy(args):
@@ -443,6 +501,91 @@ def bench_copy(args):
                          "frac": bytes_ / k / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": "copy",
                          "kernel_avg_ms": k * 1e3, "alg_bytes_per_launch": bytes_}}
 
+# --------------------------------------------------------------------------- differentiable SVD
+def bench_grad(args):
+    """Forward (batched Jacobi SVD) plus the hand-written adjoint backward of
+    4096 FP64 matrices of 64 x 32, A = Q1 diag(s) Q2^T with s ~ U[1, 10] and
+    minimum relative gap 1e-3, loss sum w_i s_i + <G_U, U> + <G_V, V>."""
+    import torch
+    from vecjac_amd import autograd as AG
+    from vecjac_amd import workloads as W
+
+    nb = 4096 if args.nb == 16384 else args.nb
+    m, n = 64, 32
+    A, w, GU, GV = W.grad_batch(nb, m, n, 515)
+    At = torch.from_numpy(A).cuda()
+    wt, GUt, GVt = (torch.from_numpy(x).cuda() for x in (w, GU, GV))
+    stream = torch.cuda.current_stream()
+
+    def step():
+        X = At.detach().requires_grad_(True)
+        U, S, V = AG.jacobi_svd(X)
+        loss = (wt * S).sum() + (GUt * U).sum() + (GVt * V).sum()
+        loss.backward()
+        return X.grad
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    # the two kernels alone, with HIP events on their stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    U, S, V, info = AG._forward(At)
+    ev[1].record(stream)
+    gU, gS, gV = GUt.clone(), wt.clone(), GVt.clone()
+    ev[2].record(stream)
+    AG.svd_backward(U, S, V, gU, gS, gV)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    f_ms, b_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    # backward kernel bytes: reads U, dU (m n), V, dV (n^2), s, ds (n); writes dA (m n)
+    bwd_bytes = nb * 8 * (3 * m * n + 2 * n * n + 2 * n)
+    # gradient against torch.linalg.svd FP64 autograd on the CPU (same gauge), on a sample
+    ns = 64
+    Ac = torch.from_numpy(A[:ns]).requires_grad_(True)
+    Uc, Sc, Vh = torch.linalg.svd(Ac, full_matrices=False)
+    order = torch.argsort(Sc, dim=-1, descending=True)
+    Sc = torch.gather(Sc, -1, order)
+    Uc = torch.gather(Uc, -1, order.unsqueeze(-2).expand_as(Uc))
+    Vc = torch.gather(Vh.transpose(-2, -1), -1, order.unsqueeze(-2).expand(ns, n, n))
+    idx = Uc.abs().argmax(dim=-2, keepdim=True)
+    sgn = torch.sign(torch.gather(Uc, -2, idx))
+    Uc, Vc = Uc * sgn, Vc * sgn
+    lc = (torch.from_numpy(w[:ns]) * Sc).sum() + (torch.from_numpy(GU[:ns]) * Uc).sum() + \
+        (torch.from_numpy(GV[:ns]) * Vc).sum()
+    lc.backward()
+    gref = Ac.grad
+    rel = float((g[:ns].cpu() - gref).abs().max() / gref.abs().max())
+    out = {
+        "metric": "batched SVD forward + adjoint backward ms",
+        "value": wall * 1e3, "unit": "ms", "higher_is_better": False,
+        "ms_per_step": wall * 1e3, "dtype": "f64",
+        "config": {"workload": f"differentiable batched SVD: {nb} FP64 matrices of {m}x{n}, s ~ U[1,10] "
+                               "(min rel gap 1e-3), loss sum w s + <G_U,U> + <G_V,V>, torch.autograd",
+                   "batch": nb, "forward_kernel_ms": f_ms, "backward_kernel_ms": b_ms,
+                   "sweeps_mean": float(info.double().mean())},
+        "roofline": {"bound": "hbm", "achieved": bwd_bytes / (b_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": bwd_bytes / (b_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_svd_bwd", "kernel_avg_ms": b_ms, "alg_bytes_per_launch": bwd_bytes},
+        "grad_check": {"vs": "torch.linalg.svd FP64 autograd (CPU), same gauge", "matrices": ns,
+                       "max_rel_err": rel},
+    }
+    if not args.no_cpu:
+        # the reference's forward (dvjsvd_) on the host; the reference has no backward
+        m_real = np.full(nb, m, dtype=np.int64)
+        n_real = np.full(nb, n, dtype=np.int64)
+        from vecjac_amd import batched as BT
+        mm, npad, offG, _, _, _ = BT.pack_shapes(m_real, n_real)
+        G64 = np.ascontiguousarray(np.transpose(A, (0, 2, 1))).reshape(-1)
+        out["cpu_baseline"] = _bsvd_cpu(m_real, n_real, mm, offG, npad, G64, "d")
+        out["cpu_baseline"]["note"] = "forward only (dvjsvd_ per matrix); the reference has no SVD backward"
+    return out
+
 
 WORKLOADS = {
     "copy": bench_copy,
@@ -450,6 +593,7 @@ WORKLOADS = {
     "zsvd": lambda a: bench_plain(a, "z"),
     "bsvd": lambda a: bench_bsvd(a, "d"),
     "bsvd32": lambda a: bench_bsvd(a, "s"),
+    "grad": bench_grad,
     "bjac2": lambda a: bench_bjac2(a, "d"),
     "zbjac2": lambda a: bench_bjac2(a, "z"),
     "sbjac2": lambda a: bench_bjac2(a, "s"),
@@ -473,6 +617,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=0, help="columns of the plain workloads (0: the config's)")
     ap.add_argument("--lg", type=int, default=26, help="log2 batch size of the 2x2 workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-tune", action="store_true", help="skip the batched plan's autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "

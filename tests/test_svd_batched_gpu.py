@@ -132,3 +132,110 @@ def test_batched_nonfinite_and_zero_members():
     r = _run("d", m, npad, G, tot)
     assert r["info"][1] == -80 and r["info"][3] == 0
     assert r["info"][0] >= 1 and r["info"][2] >= 1
+
+
+def test_batched_wide_rows_vs_plain_and_oracle():
+    """Widths 64..128 and up to 512 rows: the register footprints of 4 and 8
+    row chunks per lane (two pairs per wave, with and without prefetch) against
+    the plain driver bit for bit, and two members against the oracle."""
+    import torch
+    g = W.rng(321)
+    n = np.array([128, 96, 64, 128, 80, 112], dtype=np.int64)
+    m = np.array([512, 300, 256, 130, 500, 448], dtype=np.int64)
+    _, npad, offG, offV, offS, tot = BT.pack_shapes(m, n)
+    G = BT.fill_gaussian(m, npad, n, offG, 322)
+    r = _run("d", m, npad, G, tot)
+    assert r["rc"] == 0 and np.all(r["info"] > 0)
+    for b in range(len(m)):
+        mm, nn = int(m[b]), int(npad[b])
+        A = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
+        js, stp = S.pjs(2, nn)
+        Gd = torch.from_numpy(A.copy()).cuda()
+        Vd = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
+        eS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        fS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        jsd = torch.from_numpy(js.reshape(-1).view(np.int32)).cuda()
+        print(f"plain m={mm} n={nn}", flush=True)
+        rc, _ = S.device_vjsvd("d", Gd, Vd, eS, fS, jsd, stp)
+        assert rc == int(r["info"][b])
+        assert np.array_equal(Gd.cpu().numpy().reshape(-1), r["G"][offG[b]: offG[b] + mm * nn], equal_nan=True)
+        assert np.array_equal(Vd.cpu().numpy().reshape(-1), r["V"][offV[b]: offV[b] + nn * nn])
+        assert np.array_equal(eS.cpu().numpy(), r["eS"][offS[b]: offS[b] + nn])
+        assert np.array_equal(fS.cpu().numpy(), r["fS"][offS[b]: offS[b] + nn])
+    for b in (1, 3):
+        mm, nn = int(m[b]), int(npad[b])
+        js, stp = S.pjs(2, nn)
+        Go = vjo.aligned((nn, mm), np.float64)
+        Go[...] = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
+        with vjo.norm_mode(1):
+            rc, eSo, fSo, Vo, _, st = vjo.vjsvd_core("d", Go, js=js, stp=stp, swp=999)
+        assert rc == int(r["info"][b])
+        assert np.array_equal(eSo, r["eS"][offS[b]: offS[b] + nn])
+        assert np.array_equal(fSo, r["fS"][offS[b]: offS[b] + nn])
+
+
+def test_batched_graph_replay():
+    """The run captured in a HIP graph (its queue is reset by a fill kernel,
+    not a memset node) replays to the eager result, replay after replay, with
+    eager work on the same buffers in between."""
+    import torch
+    m, n, npad, offG, offV, offS, tot, G = _batch(40, 808, nmax=64)
+    plan = BT.BatchedSVD(m, npad, kind="d")
+    G0 = torch.from_numpy(G).cuda()
+    bufs = lambda: (torch.empty_like(G0), torch.zeros(tot[1], dtype=torch.float64, device="cuda"),
+                    torch.zeros(tot[2], dtype=torch.float64, device="cuda"),
+                    torch.zeros(tot[2], dtype=torch.float64, device="cuda"),
+                    torch.zeros(len(m), dtype=torch.int64, device="cuda"))
+    Ge, Ve, eSe, fSe, ie = bufs()
+    Ge.copy_(G0)
+    plan.run(Ge, Ve, eSe, fSe, ie)
+    torch.cuda.synchronize()
+    ref = [t.cpu().numpy() for t in (Ge, Ve, eSe, fSe, ie)]
+    Gg, Vg, eSg, fSg, ig = bufs()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        Gg.copy_(G0)
+        plan.run(Gg, Vg, eSg, fSg, ig)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        Gg.copy_(G0)
+        plan.run(Gg, Vg, eSg, fSg, ig)
+    for rep in range(3):
+        for t in (Gg, Vg, eSg, fSg):
+            t.fill_(float(rep) + 0.5)
+        ig.fill_(-7)
+        torch.cuda.synchronize()
+        print(f"graph replay {rep}", flush=True)
+        graph.replay()
+        torch.cuda.synchronize()
+        got = [t.cpu().numpy() for t in (Gg, Vg, eSg, fSg, ig)]
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b, equal_nan=True), rep
+    plan.close()
+
+
+def test_batched_autotune_keeps_results():
+    """The autotuner leaves G untouched, picks a launch shape, and the tuned
+    plan computes the same bits as any other shape."""
+    import torch
+    m, n, npad, offG, offV, offS, tot, G = _batch(32, 909, nmax=64)
+    base = _run("d", m, npad, G, tot, waves=4)
+    plan = BT.BatchedSVD(m, npad, kind="d")
+    G0 = torch.from_numpy(G).cuda()
+    V = torch.zeros(tot[1], dtype=torch.float64, device="cuda")
+    eS = torch.zeros(tot[2], dtype=torch.float64, device="cuda")
+    fS = torch.zeros(tot[2], dtype=torch.float64, device="cuda")
+    info = torch.zeros(len(m), dtype=torch.int64, device="cuda")
+    times = plan.autotune(G0, V, eS, fS, info)
+    print(f"autotune {times} -> {plan.waves} waves", flush=True)
+    assert plan.waves in times and len(times) >= 3
+    assert np.array_equal(G0.cpu().numpy(), G)
+    Gd = G0.clone()
+    plan.run(Gd, V, eS, fS, info)
+    torch.cuda.synchronize()
+    assert np.array_equal(Gd.cpu().numpy(), base["G"], equal_nan=True)
+    assert np.array_equal(eS.cpu().numpy(), base["eS"]) and np.array_equal(fS.cpu().numpy(), base["fS"])
+    plan.close()

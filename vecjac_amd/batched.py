@@ -30,6 +30,8 @@ def _declare(L):
     L.vjh_bplan_destroy.argtypes = [C.c_void_p]
     L.vjh_bplan_waves.restype = C.c_int
     L.vjh_bplan_waves.argtypes = [C.c_void_p]
+    L.vjh_bplan_autotune.restype = C.c_int64
+    L.vjh_bplan_autotune.argtypes = [C.c_void_p] + [C.c_void_p] * 9 + [C.POINTER(C.c_float), C.c_void_p]
     L._vj_batched_declared = True
 
 
@@ -76,6 +78,26 @@ class BatchedSVD:
                              Vi.data_ptr() if Vi is not None else None, eS.data_ptr(), fS.data_ptr(),
                              info.data_ptr(), transforms.data_ptr() if transforms is not None else None, s)
         return _lib.check(rc, "vjh_bplan_run")
+
+    def autotune(self, G, V, eS, fS, info, Gi=None, Vi=None, stream=None):
+        """Time the plan at 1, 2, 4 and 8 waves per workgroup on a scratch copy
+        of G (and Gi), keep the fastest (vjh_bplan_autotune).  G is not
+        modified; V, eS, fS and info are.  Returns {waves: ms} of the runs."""
+        import torch
+        L = _lib.lib()
+        s = (stream or torch.cuda.current_stream()).cuda_stream
+        Gw = torch.empty_like(G)
+        Gwi = torch.empty_like(Gi) if Gi is not None else None
+        ms = (C.c_float * 4)()
+        rc = L.vjh_bplan_autotune(self.plan, G.data_ptr(), Gi.data_ptr() if Gi is not None else None, Gw.data_ptr(),
+                                  Gwi.data_ptr() if Gwi is not None else None, V.data_ptr(),
+                                  Vi.data_ptr() if Vi is not None else None, eS.data_ptr(), fS.data_ptr(),
+                                  info.data_ptr(), ms, s)
+        _lib.check(rc, "vjh_bplan_autotune")
+        if rc < 0:
+            raise RuntimeError(f"vjh_bplan_autotune failed ({rc}): {_lib.last_error()}")
+        self.waves = L.vjh_bplan_waves(self.plan)
+        return {w: float(ms[k]) for k, w in enumerate((1, 2, 4, 8)) if ms[k] >= 0}
 
     def close(self):
         if getattr(self, "plan", None):

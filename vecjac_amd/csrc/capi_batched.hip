@@ -22,6 +22,7 @@ struct BPlan {
   int nb = 0;
   int max_m = 0;
   int waves = 4;
+  long long totG = 0;      // elements of the packed G (max over b of offG + m n)
   unsigned swp = 999u;
   int dev = 0;
   long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each)
@@ -102,10 +103,14 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
     }
     max_m = std::max(max_m, (int)hm[b]);
   }
+  long long totG = 0;
+  for (vj_fint b = 0; b < nb; ++b)
+    totG = std::max(totG, (long long)offG[b] + hm[b] * hn[b]);
   BPlan *P = new BPlan();
   P->kind = kind;
   P->nb = (int)nb;
   P->max_m = max_m;
+  P->totG = totG;
   P->swp = swp;
   P->waves = cap_waves(kind, max_m, (waves > 0) ? waves : (nb ? pick_waves(hn) : 4));
   (void)hipGetDevice(&P->dev);
@@ -155,6 +160,57 @@ vj_fint vjh_bplan_run(void *plan, void *Gr, void *Gi, void *Vr, void *Vi, void *
 }
 
 int vjh_bplan_waves(const void *plan) { return plan ? ((const BPlan *)plan)->waves : 0; }
+
+vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
+                           void *fS, vj_fint *info, float *ms, void *stream)
+{
+  BPlan *P = (BPlan *)plan;
+  if (!P)
+    return set_error(-1, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  const bool cplx = (P->kind & 1) != 0;
+  const size_t bytes = (size_t)P->totG * ((P->kind <= 1) ? sizeof(double) : sizeof(float));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  VJ_HIP(hipEventCreate(&e0));
+  VJ_HIP(hipEventCreate(&e1));
+  static const int cand[4] = {1, 2, 4, 8};
+  int best = P->waves;
+  float best_ms = -1.0f;
+  vj_fint rc = 0;
+  for (int k = 0; k < 4 && !rc; ++k) {
+    const int w = cap_waves(P->kind, P->max_m, cand[k]);
+    float t = -1.0f;
+    if (w == cand[k]) { // a capped candidate is the same launch as a smaller one
+      P->waves = w;
+      if (bytes) {
+        if ((rc = (vj_fint)hipMemcpyAsync(Gw, G, bytes, hipMemcpyDeviceToDevice, s)))
+          rc = set_error(VJ_EHIP - (int)rc, "autotune: copy failed");
+        if (!rc && cplx && (rc = (vj_fint)hipMemcpyAsync(Gwi, Gi, bytes, hipMemcpyDeviceToDevice, s)))
+          rc = set_error(VJ_EHIP - (int)rc, "autotune: copy failed");
+      }
+      if (!rc && (rc = (vj_fint)hipEventRecord(e0, s)))
+        rc = set_error(VJ_EHIP - (int)rc, "autotune: event failed");
+      if (!rc)
+        rc = vjh_bplan_run(P, Gw, Gwi, V, Vi, eS, fS, info, nullptr, s);
+      if (!rc && (rc = (vj_fint)hipEventRecord(e1, s)))
+        rc = set_error(VJ_EHIP - (int)rc, "autotune: event failed");
+      if (!rc && (rc = (vj_fint)hipEventSynchronize(e1)))
+        rc = set_error(VJ_EHIP - (int)rc, "autotune: synchronisation failed");
+      if (!rc && (rc = (vj_fint)hipEventElapsedTime(&t, e0, e1)))
+        rc = set_error(VJ_EHIP - (int)rc, "autotune: timing failed");
+      if (!rc && (best_ms < 0.0f || t < best_ms)) {
+        best_ms = t;
+        best = w;
+      }
+    }
+    if (ms)
+      ms[k] = t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  P->waves = best;
+  return rc ? rc : (vj_fint)best;
+}
 
 void vjh_bplan_destroy(void *plan)
 {
