@@ -259,7 +259,7 @@ def _bsvd_sigma_check(m, npad, offG, offS, G64, r, kind, nsamp=6):
     for b in idx:
         mm, nn = int(m[b]), int(npad[b])
         A = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm).astype(np.float64 if kind == "d" else np.float32)
-        js, stp = S.pjs(2, nn)
+        js, stp = S.pjs(4, nn)
         Go = vjo.aligned((nn, mm), A.dtype)
         Go[...] = A
         with vjo.norm_mode(1):
@@ -283,7 +283,7 @@ def _bsvd_plain_sample(m, npad, offG, G64, kind, nsamp):
     for b in idx:
         mm, nn = int(m[b]), int(npad[b])
         A = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm).astype(dt_np)
-        js, stp = S.pjs(2, nn)
+        js, stp = S.pjs(4, nn)
         mats.append((torch.from_numpy(A).cuda(), torch.empty((nn, nn), dtype=dt, device="cuda"),
                      torch.empty(nn, dtype=dt, device="cuda"), torch.empty(nn, dtype=dt, device="cuda"),
                      torch.from_numpy(js.reshape(-1).view(np.int32)).cuda(), stp))
@@ -326,7 +326,7 @@ def _bsvd_cpu(m_real, n_real, m, offG, npad, G64, kind):
             m8 = max(m8, n16)
             A = vjo.aligned((n16, m8), np.float64)
             A[:nn, :mr] = G64[offG[b]: offG[b] + mm * int(npad[b])].reshape(int(npad[b]), mm)[:nn, :mr]
-            js, stp = S.pjs(2, n16)
+            js, stp = S.pjs(4, n16)
             jsa = vjo.aligned(js.shape, np.uint32)
             jsa[...] = js
             V = vjo.aligned((n16, n16), np.float64)
@@ -346,7 +346,7 @@ def _bsvd_cpu(m_real, n_real, m, offG, npad, G64, kind):
             mm, nn = int(m[b]), int(npad[b])
             A = vjo.aligned((nn, mm), np.float64 if kind == "d" else np.float32)
             A[...] = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
-            js, stp = S.pjs(2, nn)
+            js, stp = S.pjs(4, nn)
             rc, *_ = vjo.vjsvd_core(kind, A, js=js, stp=stp, swp=999)
             return rc
     done = 0
@@ -382,6 +382,7 @@ def bench_plain(args, kind: str):
         Ar = np.ascontiguousarray(A.real.T)
         Ai = np.ascontiguousarray(A.imag.T)
         desc = f"plain zvjsvd, complex FP64 {m}x{n}, A = Q1 diag(10^(-12 i/(n-1))) Q2^H, Haar Q1 Q2, ME table"
+    print(f"bench plain {kind}: {m}x{n} input ready", file=sys.stderr, flush=True)
     js, stp = S.pjs(2, n)
     jsd = torch.from_numpy(js.reshape(-1).view(np.int32)).cuda()
     G0 = torch.from_numpy(np.ascontiguousarray(Ar)).cuda()
@@ -404,6 +405,7 @@ def bench_plain(args, kind: str):
         if it >= args.warmup:
             walls.append(time.perf_counter() - t0)
         sweeps, hist = rc, st
+        print(f"bench plain {kind}: run {it} {time.perf_counter() - t0:.2f} s, {rc} sweeps", file=sys.stderr, flush=True)
     wall = float(np.mean(walls))
     w = 8 if kind == "d" else 16
     R = float(np.sum(hist[2][: max(0, sweeps) + 1]))

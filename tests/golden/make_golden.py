@@ -8,7 +8,8 @@ The fixtures are inputs and the reference's outputs for them:
               seeded inputs of the 2^26-batch distribution plus special values;
   dvjsvd.npz  dvjsvd_ (built with its own long-double norm dnormf_) on small
               seeded matrices and on the graded 16x16 matrix of etc/dgsex.c,
-              driven by this package's ME step table.
+              driven by this package's ME step table, which is stored with
+              each case.
 The reference itself is never copied: only these input/output vectors are.
 """
 from __future__ import annotations
@@ -60,10 +61,12 @@ def dgsex(N: int = 16) -> np.ndarray:
     return A
 
 
-def ref_dvjsvd(A: np.ndarray, swp: int = 999):
+def ref_dvjsvd(A: np.ndarray, swp: int = 999, js=None):
     R = vjo.ref()
     n, m = A.shape
-    js, stp = S.pjs(2, n)
+    if js is None:
+        js, _ = S.pjs(2, n)
+    stp = js.shape[0]
     jsa = vjo.aligned(js.shape, np.uint32)
     jsa[...] = js
     G = vjo.aligned((n, m), np.float64)
@@ -95,8 +98,10 @@ def golden_dvjsvd() -> None:
     data = {}
     for name, A in cases.items():
         swp = 2 if name.endswith("swp2") else 999
-        rc, G, V, eS, fS = ref_dvjsvd(A, swp)
+        js, _ = S.pjs(2, A.shape[0])
+        rc, G, V, eS, fS = ref_dvjsvd(A, swp, js)
         data[f"{name}_A"] = A
+        data[f"{name}_js"] = js
         data[f"{name}_swp"] = np.array(swp)
         data[f"{name}_rc"] = np.array(rc)
         data[f"{name}_U"] = G

@@ -57,7 +57,7 @@ def test_batched_vs_oracle_f64(waves):
     for b in range(nb):
         mm, nn = int(m[b]), int(npad[b])
         A = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
-        js, stp = S.pjs(2, nn)
+        js, stp = S.pjs(4, nn)
         Go = vjo.aligned((nn, mm), np.float64)
         Go[...] = A
         with vjo.norm_mode(1):
@@ -84,7 +84,7 @@ def test_batched_equals_plain_bitwise():
     for b in range(nb):
         mm, nn = int(m[b]), int(npad[b])
         A = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
-        js, stp = S.pjs(2, nn)
+        js, stp = S.pjs(4, nn)
         Gd = torch.from_numpy(A.copy()).cuda()
         Vd = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
         eS = torch.empty(nn, dtype=torch.float64, device="cuda")
@@ -149,7 +149,7 @@ def test_batched_wide_rows_vs_plain_and_oracle():
     for b in range(len(m)):
         mm, nn = int(m[b]), int(npad[b])
         A = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
-        js, stp = S.pjs(2, nn)
+        js, stp = S.pjs(4, nn)
         Gd = torch.from_numpy(A.copy()).cuda()
         Vd = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
         eS = torch.empty(nn, dtype=torch.float64, device="cuda")
@@ -164,7 +164,7 @@ def test_batched_wide_rows_vs_plain_and_oracle():
         assert np.array_equal(fS.cpu().numpy(), r["fS"][offS[b]: offS[b] + nn])
     for b in (1, 3):
         mm, nn = int(m[b]), int(npad[b])
-        js, stp = S.pjs(2, nn)
+        js, stp = S.pjs(4, nn)
         Go = vjo.aligned((nn, mm), np.float64)
         Go[...] = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
         with vjo.norm_mode(1):

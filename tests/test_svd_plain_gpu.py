@@ -108,6 +108,8 @@ def _checks(kind, A, Ai, r, tolmul=8.0):
     ("z", 96, 32), ("z", 600, 24), ("z", 1500, 16),
     ("s", 100, 32), ("s", 700, 32),
     ("c", 80, 32), ("c", 1100, 16),
+    # columns streamed in chunks (longer than the register-held pairs)
+    ("z", 5000, 16), ("d", 9000, 16), ("c", 4500, 8), ("s", 8500, 8),
 ])
 def test_plain_vs_oracle(kind, m, n):
     A, Ai = _mat(kind, m, n, seed=m * 7 + n)

@@ -1,8 +1,8 @@
 // C-ABI of the batched SVD (no reference counterpart): a plan holds the
 // shape-dependent metadata on the device (descriptors and the cost order),
 // so that a run is a single kernel launch on resident data and can be
-// captured in a HIP graph.  The kernel computes the round-robin pairs of
-// each step itself (the pairs of pjs(2, n)).
+// captured in a HIP graph.  The kernel computes the pairs of each step
+// itself: the modulus ordering, the table pjs(4, n).
 #include "svd_batched.hpp"
 #include "svd_common.hpp"
 #include "vecjac.h"

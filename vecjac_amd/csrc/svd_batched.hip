@@ -5,19 +5,16 @@
 // Workgroups pull matrices from a queue sorted by decreasing cost (m n^2),
 // so the long ones start first and the short ones fill the tail.  A
 // workgroup of W waves owns one matrix at a time and runs every sweep of it
-// without leaving the kernel.  Each step of the strategy table runs in three
-// phases separated by workgroup barriers:
-//   A  the waves share out the n/2 pairs; a wave loads its pair, computes the
-//      two norms and the dot product with wave-level reductions and leaves
-//      the (e, f) norms and the off-diagonal element in LDS;
-//   B  one lane per pair takes the rotate/swap/skip decision and solves the
-//      2x2 EVD, so the divisions and square roots run on all lanes at once
-//      instead of once per wave;
-//   C  the waves reload their pairs, apply the transformations to G and V
-//      and write the new column norms.
-// The per-matrix control state (max-|.| bounds, shifts, transformation
-// counts) is folded through double-buffered LDS slots, so every thread takes
-// the same rescaling and convergence decisions; no host round trip.
+// without leaving the kernel.  The pairs of a step (the modulus ordering,
+// computed, not read from a table) are dealt to the waves; a wave takes its
+// pairs two at a time (wave_step): the next two are loaded while these are
+// reduced (norms and dot products with DPP / permlane wave reductions), lane
+// r decides pair r (convergence test, 2x2 EVD), and the decisions are
+// applied to G and V in registers.  One workgroup barrier ends a step; the
+// per-matrix control state (max-|.| bounds, shifts, transformation counts)
+// is folded through double-buffered LDS slots, so every thread takes the
+// same rescaling and convergence decisions; no host round trip.  The
+// register footprint (64-row chunks per lane) is chosen per matrix.
 #include "svd_common.hpp"
 #include "svd_batched.hpp"
 #include "pjs.hpp"
@@ -247,7 +244,7 @@ template <typename T> __device__ __forceinline__ PairDec<T> lane_dec(const PairD
 }
 
 // One step of the strategy for the pairs of one wave (pairs wave, wave + W,
-// wave + 2W, ... of the round-robin table of width n), GP pairs at a time:
+// wave + 2W, ... of the modulus table of width n, pjs id 4), GP pairs at a time:
 // load (the next group is prefetched), reduce the GP pairs together, decide
 // with lane r of every quad working on pair r, broadcast each decision and
 // apply it in registers, store.  Accumulates this wave's per-lane max of the
@@ -270,7 +267,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
   auto pair_of = [&](const int g, const int r, unsigned &pp, unsigned &qq) {
     int i = g * GP + r;
     i = (i < cnt) ? i : (cnt - 1);
-    rr_pair((unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
+    mod_pair((unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
   };
   if constexpr (PF) {
 #pragma unroll
@@ -319,6 +316,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
     for (int r = 0; r < GP; ++r) {
       kx[r] = max_exp(mx[r]);
       ky[r] = max_exp(my[r]);
+      sx[r] = sy[r] = dr[r] = di[r] = DD<A>{0, 0};
       pair_sums<T, CPLX, E>(xr[r], yr[r], xi[r], yi[r], nE, kx[r], ky[r], sx[r], sy[r], dr[r], di[r]);
       red_dd4<A, B>(sx[r], sy[r], dr[r], di[r], nullptr);
     }
@@ -429,7 +427,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     T *const Vi = CPLX ? a.Vi + a.offV[b] : nullptr;
     T *const eS = a.eS + a.offS[b];
     T *const fS = a.fS + a.offS[b];
-    const unsigned stp = (unsigned)(n - 1);
+    const unsigned stp = (unsigned)n; // steps per sweep of the modulus ordering
     const int nE = (m + 63) >> 6;
 
     // ---- setup (dvjsvd.c:78-184): max |G|, V := I, eS/fS, initial scale

@@ -369,8 +369,9 @@ __device__ __forceinline__ void pair_maxima(const T (&xr)[E], const T (&yr)[E], 
 // Scaling exponent of a column from its max |.| (0 for a zero column).
 template <typename T> __device__ __forceinline__ int max_exp(const T mx) { return (mx > (T)0) ? (frexp_exp(mx) - 1) : 0; }
 
-// pair_sums: partial double-double sums of |x 2^-kx|^2, |y 2^-ky|^2 and
-// x^H y 2^-(kx+ky) (real part dr, imaginary part di) over this lane's rows.
+// pair_sums: adds to the lane's double-double sums of |x 2^-kx|^2,
+// |y 2^-ky|^2 and x^H y 2^-(kx+ky) (real part dr, imaginary part di) over
+// its rows, in row order (the caller starts them at zero).
 template <typename T, bool CPLX, int E>
 __device__ __forceinline__ void pair_sums(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E],
                                           const int nE, const int kx, const int ky, DD<typename Acc<T>::A> &sx,
@@ -378,7 +379,6 @@ __device__ __forceinline__ void pair_sums(const T (&xr)[E], const T (&yr)[E], co
                                           DD<typename Acc<T>::A> &di)
 {
   typedef typename Acc<T>::A A;
-  sx = sy = dr = di = DD<A>{0, 0};
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     if (j >= nE)
@@ -459,7 +459,7 @@ pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[
   pair_maxima<T, CPLX, E>(xr, yr, xi, yi, nE, mx, my);
   red_max2<T, B>(mx, my, lds);
   const int kx = max_exp(mx), ky = max_exp(my);
-  DD<A> sx, sy, dr, di;
+  DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
   pair_sums<T, CPLX, E>(xr, yr, xi, yi, nE, kx, ky, sx, sy, dr, di);
   red_dd4<A, B>(sx, sy, dr, di, lds);
   return pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);

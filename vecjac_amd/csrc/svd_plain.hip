@@ -119,7 +119,60 @@ __global__ __launch_bounds__(256) void k_init_cols(const int m, const int n, T *
 }
 
 // ------------------------------------------------------------- step kernel
+// Rows lane + j B (j < E) of the pair's columns, zero past m, scaled by
+// 2^sG when a rescaling is pending.
 template <typename T, bool CPLX, int B, int E>
+__device__ __forceinline__ void load_rows(const T *Gp, const T *Gq, const T *Gpi, const T *Gqi, const int m, const int j0,
+                                          const int sG, const int lane, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E])
+{
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int i = lane + (j0 + j) * B;
+    const bool in = (i < m);
+    xr[j] = in ? Gp[i] : (T)0;
+    yr[j] = in ? Gq[i] : (T)0;
+    if constexpr (CPLX) {
+      xi[j] = in ? Gpi[i] : (T)0;
+      yi[j] = in ? Gqi[i] : (T)0;
+    }
+    else {
+      xi[j] = yi[j] = (T)0;
+    }
+    if (sG) {
+      xr[j] = ldexp_(xr[j], sG);
+      yr[j] = ldexp_(yr[j], sG);
+      if constexpr (CPLX) {
+        xi[j] = ldexp_(xi[j], sG);
+        yi[j] = ldexp_(yi[j], sG);
+      }
+    }
+  }
+}
+
+template <typename T, bool CPLX, int B, int E>
+__device__ __forceinline__ void store_rows(T *Gp, T *Gq, T *Gpi, T *Gqi, const int m, const int j0, const int lane,
+                                           const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E])
+{
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int i = lane + (j0 + j) * B;
+    if (i < m) {
+      Gp[i] = xr[j];
+      Gq[i] = yr[j];
+      if constexpr (CPLX) {
+        Gpi[i] = xi[j];
+        Gqi[i] = yi[j];
+      }
+    }
+  }
+}
+
+// One step on pair blockIdx.x.  STREAM = false: the pair is held in
+// registers, E rows per lane.  STREAM = true (columns too long for the
+// registers): the pair is read three times in chunks of E rows per lane -
+// maxima, sums, update - with the same per-lane row order, hence the same
+// bits as the register form with the same group size.
+template <typename T, bool CPLX, int B, int E, bool STREAM = false>
 __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restrict__ Gr, T *__restrict__ Gi,
                                              const long long ldG, T *__restrict__ Vr, T *__restrict__ Vi,
                                              const long long ldV, T *__restrict__ eS, T *__restrict__ fS,
@@ -143,29 +196,6 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
 
   T *const Gp = Gr + (long long)p * ldG, *const Gq = Gr + (long long)q * ldG;
   T *const Gpi = CPLX ? Gi + (long long)p * ldG : nullptr, *const Gqi = CPLX ? Gi + (long long)q * ldG : nullptr;
-  T xr[E], yr[E], xi[E], yi[E];
-#pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const int i = lane + j * B;
-    const bool in = (i < m);
-    xr[j] = in ? Gp[i] : (T)0;
-    yr[j] = in ? Gq[i] : (T)0;
-    if constexpr (CPLX) {
-      xi[j] = in ? Gpi[i] : (T)0;
-      yi[j] = in ? Gqi[i] : (T)0;
-    }
-    else {
-      xi[j] = yi[j] = (T)0;
-    }
-    if (S.sG) {
-      xr[j] = ldexp_(xr[j], S.sG);
-      yr[j] = ldexp_(yr[j], S.sG);
-      if constexpr (CPLX) {
-        xi[j] = ldexp_(xi[j], S.sG);
-        yi[j] = ldexp_(yi[j], S.sG);
-      }
-    }
-  }
   T *const Vp = Vr + (long long)p * ldV, *const Vq = Vr + (long long)q * ldV;
   T *const Vpi = CPLX ? Vi + (long long)p * ldV : nullptr, *const Vqi = CPLX ? Vi + (long long)q * ldV : nullptr;
   if (S.sV) {
@@ -179,19 +209,53 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     }
   }
   T eSp, fSp, eSq, fSq;
-  const PairOut<T, CPLX> o = pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, lds, lane);
-  if (!o.err && ((o.code != 1) || S.sG)) {
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int i = lane + j * B;
-      if (i < m) {
-        Gp[i] = xr[j];
-        Gq[i] = yr[j];
-        if constexpr (CPLX) {
-          Gpi[i] = xi[j];
-          Gqi[i] = yi[j];
-        }
-      }
+  PairOut<T, CPLX> o;
+  if constexpr (!STREAM) {
+    T xr[E], yr[E], xi[E], yi[E];
+    load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, S.sG, lane, xr, yr, xi, yi);
+    o = pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, lds, lane);
+    if (!o.err && ((o.code != 1) || S.sG))
+      store_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, lane, xr, yr, xi, yi);
+  }
+  else {
+    typedef typename Acc<T>::A A;
+    const int nc = (m + B * E - 1) / (B * E); // chunks of E rows per lane
+    T xr[E], yr[E], xi[E], yi[E];
+    T mx = (T)0, my = (T)0;
+    for (int c = 0; c < nc; ++c) {
+      load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
+      T cx, cy;
+      pair_maxima<T, CPLX, E>(xr, yr, xi, yi, E, cx, cy);
+      mx = __builtin_fmax(mx, cx);
+      my = __builtin_fmax(my, cy);
+    }
+    red_max2<T, B>(mx, my, lds);
+    const int kx = max_exp(mx), ky = max_exp(my);
+    DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
+    for (int c = 0; c < nc; ++c) {
+      load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
+      pair_sums<T, CPLX, E>(xr, yr, xi, yi, E, kx, ky, sx, sy, dr, di);
+    }
+    red_dd4<A, B>(sx, sy, dr, di, lds);
+    const PairScal<T> P = pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
+    const PairDec<T> D = pair_decide<T, CPLX>(P, S.tol, S.gst);
+    T tG = (T)0;
+    for (int c = 0; c < nc; ++c) {
+      load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
+      // the V pair is updated with the first chunk
+      o = pair_apply<T, CPLX, B, E, false>(c ? 0 : n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, S.vtrack, eSp, fSp, eSq,
+                                           fSq, lds, lane, E);
+      if (o.err)
+        break;
+      tG = xmax(tG, o.tG);
+      if ((o.code != 1) || S.sG)
+        store_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, lane, xr, yr, xi, yi);
+    }
+    if (!o.err) {
+      tG = red_max<T, B>(tG, lds);
+      o.tG = tG;
+      if (((o.code == 2) || (o.code == -2) || (o.code == 3) || (o.code == -3)) && !isfinite_(tG))
+        o.err = CPLX ? -630 : -580;
     }
   }
   if (lane == 0) {
@@ -297,10 +361,10 @@ __global__ __launch_bounds__(256) void k_epilogue(const int m, const int n, T *_
 }
 
 // ------------------------------------------------------------------ driver
-template <typename T, bool CPLX, int B, int E>
+template <typename T, bool CPLX, int B, int E, bool STREAM = false>
 static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, const unsigned *row, unsigned s, const T tol, const T gst, hipStream_t strm)
 {
-  hipLaunchKernelGGL((k_step<T, CPLX, B, E>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m, (int)a.n, a.Gr, a.Gi, a.ldG,
+  hipLaunchKernelGGL((k_step<T, CPLX, B, E, STREAM>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m, (int)a.n, a.Gr, a.Gi, a.ldG,
                      a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp, tol, gst, a.vpreset, st);
   return 0;
 }
@@ -312,16 +376,21 @@ template <typename T, bool CPLX>
 static StepFn<T, CPLX> pick_step(long long m, int &B_out)
 {
   // smallest group that holds the column pair in registers with <= 8
-  // elements per lane (<= 16 for the largest group)
+  // elements per lane, then 512 lanes (two waves per SIMD, 256 registers per
+  // lane) with 8 or 16; longer columns are streamed through 512 lanes in
+  // chunks of 4 rows per lane (a 1024-lane group would cap the registers at
+  // 128 and spill the pair)
   if (m <= 64) { B_out = 64; return &launch_steps<T, CPLX, 64, 1>; }
   if (m <= 128) { B_out = 64; return &launch_steps<T, CPLX, 64, 2>; }
   if (m <= 256) { B_out = 64; return &launch_steps<T, CPLX, 64, 4>; }
   if (m <= 512) { B_out = 64; return &launch_steps<T, CPLX, 64, 8>; }
   if (m <= 1024) { B_out = 256; return &launch_steps<T, CPLX, 256, 4>; }
   if (m <= 2048) { B_out = 256; return &launch_steps<T, CPLX, 256, 8>; }
-  if (m <= 4096) { B_out = 1024; return &launch_steps<T, CPLX, 1024, 4>; }
-  if (m <= 8192) { B_out = 1024; return &launch_steps<T, CPLX, 1024, 8>; }
-  if (m <= 16384) { B_out = 1024; return &launch_steps<T, CPLX, 1024, 16>; }
+  if (m <= 4096) { B_out = 512; return &launch_steps<T, CPLX, 512, 8>; }
+  if constexpr (!CPLX) {
+    if (m <= 8192) { B_out = 512; return &launch_steps<T, CPLX, 512, 16>; }
+  }
+  if (m <= 0x7fffffffLL / 2) { B_out = 512; return &launch_steps<T, CPLX, 512, 4, true>; }
   return nullptr;
 }
 
@@ -333,7 +402,7 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   int B = 0;
   StepFn<T, CPLX> step = pick_step<T, CPLX>(a.m, B);
   if (!step)
-    return set_error(-1, "m = %lld exceeds the largest supported column length (16384)", a.m);
+    return set_error(-1, "m = %lld exceeds the largest supported column length", a.m);
   PlainState<T> *st = nullptr;
   VJ_HIP(hipMallocAsync((void **)&st, sizeof(PlainState<T>), s));
   VJ_HIP(hipMemsetAsync(st, 0, sizeof(PlainState<T>), s));
