@@ -405,6 +405,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
 {
   constexpr int B = 64;
   __shared__ int s_b;
+  __shared__ T s_eS[128], s_fS[128];           // the matrix's (e,f) column norms (n <= 128)
   __shared__ double s_max[2][W][2];            // per-wave (tG, tV) maxima, by step parity
   __shared__ unsigned long long s_cnt[2][W];   // per-wave transformation counts, by step parity
   __shared__ int s_err[2][W];
@@ -458,10 +459,11 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           Vi[t] = (T)0;
       }
       for (int j = threadIdx.x; j < n; j += 64 * W) {
-        eS[j] = -inf_<T>();
-        fS[j] = (T)1;
+        s_eS[j] = -inf_<T>();
+        s_fS[j] = (T)1;
       }
     }
+    const bool norms = !status; // eS/fS are written back (from LDS) at the end
     unsigned sw = 0;
     unsigned long long total = 0;
     int sT = 0, NRM_EXP = 0;
@@ -497,15 +499,15 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           // register footprint by this matrix's row count (chunks of 64 rows)
           if ((E >= 2) && (nE <= 2))
             wave_step<T, CPLX, (E >= 2 ? 2 : E), W, StepShape<T, CPLX, (E >= 2 ? 2 : E)>::GP, StepShape<T, CPLX, (E >= 2 ? 2 : E)>::PF>(
-                Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
+                Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
           else if ((E >= 4) && (nE <= 4))
             wave_step<T, CPLX, (E >= 4 ? 4 : E), W, StepShape<T, CPLX, (E >= 4 ? 4 : E)>::GP, StepShape<T, CPLX, (E >= 4 ? 4 : E)>::PF>(
-                Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
+                Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
           else if ((E >= 8) && (nE <= 8))
             wave_step<T, CPLX, (E >= 8 ? 8 : E), W, StepShape<T, CPLX, (E >= 8 ? 8 : E)>::GP, StepShape<T, CPLX, (E >= 8 ? 8 : E)>::PF>(
-                Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
+                Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
           else
-            wave_step<T, CPLX, E, W, StepShape<T, CPLX, E>::GP, StepShape<T, CPLX, E>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st, wave, lane, S,
+            wave_step<T, CPLX, E, W, StepShape<T, CPLX, E>::GP, StepShape<T, CPLX, E>::PF>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, nE, st, wave, lane, S,
                                                                    wG, wcnt, werr);
           wG = red_max<T, 64>(wG, nullptr);
           if (lane == 0) {
@@ -562,16 +564,22 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       if (!status && (sw < a.swp)) {
         for (long long t = threadIdx.x; t < (long long)m * n; t += 64 * W) {
           const int j = (int)(t / m);
-          const T f_ = fS[j], s_ = -eS[j];
+          const T f_ = s_fS[j], s_ = -s_eS[j];
           Gr[t] = scalef(Gr[t] / f_, s_);
           if constexpr (CPLX)
             Gi[t] = scalef(Gi[t] / f_, s_);
         }
         __syncthreads();
         for (int j = threadIdx.x; j < n; j += 64 * W)
-          eS[j] -= (T)sT;
+          s_eS[j] -= (T)sT;
       }
     }
+    __syncthreads();
+    if (norms)
+      for (int j = threadIdx.x; j < n; j += 64 * W) {
+        eS[j] = s_eS[j];
+        fS[j] = s_fS[j];
+      }
     if (threadIdx.x == 0) {
       a.info[b] = status ? (long long)status : (long long)sw;
       if (a.transforms)
