@@ -29,6 +29,13 @@
 
 typedef int64_t vj_fint;   /* fint: long long under MKL_ILP64 (common.h:184-186) */
 typedef uint64_t vj_fnat;  /* fnat: unsigned long long (common.h:185-186) */
+/* element type of interleaved complex arrays: double complex in C (as the
+ * reference's prototypes), one (re, im) pair of doubles seen from C++ */
+#ifdef __cplusplus
+typedef double vj_zdouble;
+#else
+typedef double _Complex vj_zdouble;
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -75,6 +82,15 @@ vj_fint cvjsvd_(const vj_fnat *m, const vj_fnat *n, float *Gr, const vj_fnat *ld
  * id 2 (ME): n-1 steps; id 4 (MM): n steps; n even.  Returns a malloc'd
  * stp x n table (free with free() or pjs_free()), NULL on bad arguments. */
 unsigned *pjs(long id, unsigned n, unsigned *stp);
+
+/* Interleaved -> split complex layout; replaces zsplit_ (zsplit.h:6,
+ * zsplit.c:3): A (m x n complex, ld ldA) into Ar, Ai (ld ldAr, ldAi).
+ * Returns 0, or -4/-6/-8 for a leading dimension below m. */
+vj_fint zsplit_(const vj_fnat *m, const vj_fnat *n, const vj_zdouble *A, const vj_fnat *ldA, double *Ar, const vj_fnat *ldAr,
+                double *Ai, const vj_fnat *ldAi);
+/* Split -> interleaved; replaces zmerge_ (zmerge.h:6, zmerge.c:3). */
+vj_fint zmerge_(const vj_fnat *m, const vj_fnat *n, const double *Ar, const vj_fnat *ldAr, const double *Ai,
+                const vj_fnat *ldAi, vj_zdouble *A, const vj_fnat *ldA);
 void pjs_free(unsigned *tab);
 
 /* ---- 2. device-memory extensions ---------------------------------------- */
@@ -135,6 +151,13 @@ int vjh_bplan_waves(const void *plan);
 vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
                            void *fS, vj_fint *info, float *ms, void *stream);
 void vjh_bplan_destroy(void *plan);
+/* zsplit_ / zmerge_ on device memory (interleaved A as pairs of doubles),
+ * one coalesced pass on `stream`, no host round trip. */
+vj_fint vjh_zsplit(vj_fnat m, vj_fnat n, const double *A, vj_fnat ldA, double *Ar, vj_fnat ldAr, double *Ai, vj_fnat ldAi,
+                   void *stream);
+vj_fint vjh_zmerge(vj_fnat m, vj_fnat n, const double *Ar, vj_fnat ldAr, const double *Ai, vj_fnat ldAi, double *A, vj_fnat ldA,
+                   void *stream);
+
 /* One-shot FP64 batched SVD (plan, run, synchronise, destroy). */
 vj_fint vjh_dbvjsvd(vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG, const vj_fint *offV,
                     const vj_fint *offS, double *G, double *V, double *eS, double *fS, vj_fint *info,

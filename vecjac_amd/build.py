@@ -21,6 +21,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libvecjac_hip.so"
+DRIVERS = PKG / "drivers"
+BIN = PKG / "bin"
 ARCH = os.environ.get("VJ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -60,6 +62,24 @@ def build_hip(verbose: bool = True) -> Path:
     return LIB
 
 
+def build_drivers(verbose: bool = True) -> list:
+    """The command-line drivers (drivers/*.c, the reference's tdvjsvd /
+    tzvjsvd / tdbjac2 interfaces) into bin/, linked against the library."""
+    BIN.mkdir(exist_ok=True)
+    out = []
+    for src in sorted(DRIVERS.glob("*.c")):
+        exe = BIN / src.stem
+        cmd = ["gcc", "-O2", "-std=gnu11", "-Wall", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+               f"-L{LIB.parent}", "-lvecjac_hip", "-Wl,-rpath,$ORIGIN/..", "-lm"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        out.append(exe)
+    if verbose:
+        print(f"built {len(out)} drivers in {BIN}")
+    return out
+
+
 def build_oracle(verbose: bool = True) -> None:
     r = subprocess.run(["make", "-C", str(ROOT / "oracle"), "-j8"], capture_output=True, text=True)
     if r.returncode != 0:
@@ -70,6 +90,7 @@ def build_oracle(verbose: bool = True) -> None:
 
 def main() -> int:
     build_hip()
+    build_drivers()
     build_oracle()
     return 0
 
