@@ -66,6 +66,13 @@ vj_fint cbjac2_(const vj_fint *n, const float *a11, const float *a22, const floa
 vj_fint dvjsvd_(const vj_fnat *m, const vj_fnat *n, double *G, const vj_fnat *ldG, double *V, const vj_fnat *ldV,
                 double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work,
                 unsigned *iwork);
+/* Trace switch (the reference's compile-time JTRACE, dvjsvd.c:70-635): with
+ * on != 0 the host entry points ?vjsvd_ write the reference's trace file,
+ * named by the string in `work`, in the reference's line format; the
+ * per-phase times Tn..Te are 0 (one fused GPU step) and Tr is the GPU time
+ * of the sweeps.  Process-wide; off by default. */
+void vjh_set_jtrace(int on);
+
 /* Complex (split re/im) twin; replaces zvjsvd_ (zvjsvd.h:10, zvjsvd.c:40). */
 vj_fint zvjsvd_(const vj_fnat *m, const vj_fnat *n, double *Gr, const vj_fnat *ldGr, double *Gi, const vj_fnat *ldGi,
                 double *Vr, const vj_fnat *ldVr, double *Vi, const vj_fnat *ldVi, double *eS, double *fS,

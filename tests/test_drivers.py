@@ -178,3 +178,57 @@ def test_tdbjac2_driver(tmp_path):
         # the inputs were rounded to double after the construction: the
         # eigenvalues move by about eps |A| / |lambda|
         assert float(RLN) <= 1e-2
+
+
+def _traces(tmp_path, A, name):
+    import os
+    from vecjac_amd import svd as S
+    n, m = A.shape
+    base = tmp_path / name
+    A.tofile(str(base) + ".G")
+    r = subprocess.run([str(BIN / "vj_tdvjsvd"), "2", str(m), str(n), str(base)], capture_output=True, text=True,
+                       timeout=300, env={**os.environ, "VJ_JTRACE": "1"})
+    assert r.returncode == 0, r.stderr
+    ours = Path(f"{base}.2").read_text().splitlines()
+    js, _ = S.pjs(2, n)
+    js.astype(np.uint32).tofile(str(base) + ".js")
+    rr = subprocess.run([str(ROOT / "oracle" / "_ref" / "ref_trace"), str(m), str(n), str(base) + ".G",
+                         str(base) + ".js", str(base) + ".ref"], capture_output=True, text=True, timeout=300)
+    assert rr.returncode == 0, rr.stderr
+    ref = Path(f"{base}.ref").read_text().splitlines()
+    print("\n".join(["ours:"] + ours + ["reference:"] + ref))
+    return ours, ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["gauss64x32", "dgsex16"])
+def test_jtrace_vs_reference(tmp_path, case):
+    """The trace file of vj_tdvjsvd with VJ_JTRACE set against the trace of
+    the reference's own JTRACE build on the same input and table: the same
+    lines (initial max, initial scaling, rescalings with their sweep and
+    step, transformations per sweep, final shift and bound), except the
+    per-phase times of the last line."""
+    if not (ROOT / "oracle" / "_ref" / "ref_trace").exists():
+        pytest.skip("reference build absent")
+    if case == "dgsex16":
+        from tests.golden.make_golden import dgsex
+        A = dgsex(16)
+    else:
+        A = W.gaussian_matrix(64, 32, 44)
+    ours, ref = _traces(tmp_path, A, case)
+    assert len(ours) == len(ref)
+    assert ours[:2] == ref[:2]                       # max |G| and the initial scaling: exact
+    num = lambda s: float(s.split("M=")[1])
+    for a, b in zip(ours[2:-1], ref[2:-1]):
+        if a.startswith("sweep=") or a.startswith("sT="):
+            # sweep, step, eM, sR, sN (or sT) exact; the bound M after the
+            # shift is a max over updated elements: last-bit differences
+            assert a.split("M=")[0] == b.split("M=")[0]
+            assert abs(num(a) - num(b)) <= 1e-12 * abs(num(b))
+        else:
+            # "sweep transformations": the norms here are correctly rounded and
+            # the reference's are its own, so a pair sitting on the tolerance
+            # can go either way
+            (sa, ta), (sb, tb) = map(int, a.split()), map(int, b.split())
+            assert sa == sb and abs(ta - tb) <= max(2, tb // 100)
+    assert ours[-1].startswith("Tn=") and ref[-1].startswith("Tn=")

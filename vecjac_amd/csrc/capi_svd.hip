@@ -6,7 +6,9 @@
 #include "svd_common.hpp"
 #include "vecjac.h"
 
+#include <atomic>
 #include <stdint.h>
+#include <stdio.h>
 #include <string.h>
 #include <vector>
 #ifdef __x86_64__
@@ -56,6 +58,46 @@ __global__ void k_table_check(const unsigned *js, unsigned n, int *bad)
     if (v >= n || (atomicOr(&mark[v >> 5], 1u << (v & 31u)) & (1u << (v & 31u))))
       atomicOr(bad, 1);
   }
+}
+
+// JTRACE-compatible trace (the reference's compile-time JTRACE output,
+// dvjsvd.c:70-635, zvjsvd.c:83-692): switched on at run time with
+// vjh_set_jtrace(1); the host entry points then take the file name from
+// `work`, as the reference's JTRACE builds do.  The lines are the
+// reference's (initial max, initial scaling, every rescaling with the step
+// it precedes, one "sweep transformations" line per sweep, the final shift
+// and bound).  The fused GPU step has no separate norm/dot/assembly/EVD/
+// rotation phases: Tn..Te are written as 0 and Tr holds the sweeps' GPU time.
+std::atomic<int> g_jtrace{0};
+
+void write_trace(FILE *f, const PlainRun &r, int rc, bool input_error)
+{
+  (void)fprintf(f, "M=");
+  if (input_error)
+    return;
+  (void)fprintf(f, "%#.17e\n", r.MG0);
+  if (r.zero)
+    return;
+  (void)fprintf(f, "eM=%d, sR=%d, sN=%d, M=%#.17e\n", r.eM0, r.sR0, r.sN0, r.MG_scaled);
+  unsigned e = 0u;
+  for (unsigned sw = 0u; sw < 64u; ++sw) {
+    for (; e < r.nev && e < 32u && r.ev[e].sw == sw; ++e)
+      (void)fprintf(f, "sweep=%u, step=%u, eM=%d, sR=%d, sN=%d, M=%#.17e\n", r.ev[e].sw, r.ev[e].st, r.ev[e].eM,
+                    r.ev[e].sR, r.ev[e].sN, r.ev[e].M);
+    if (r.status && (r.err_sw == sw)) {
+      (void)fprintf(f, "sweep=%u, step=%u\n", r.err_sw, r.err_st);
+      return;
+    }
+    if (sw >= r.launched_sweeps)
+      break;
+    (void)fprintf(f, "%3u %llu\n", sw, r.hist[sw]);
+    if (!r.hist[sw] || (sw + 1u > r.sweeps))
+      break;
+  }
+  (void)rc;
+  (void)fprintf(f, "sT=%d, M=%#.17e\n", r.sT, r.MG_final);
+  (void)fprintf(f, "Tn=%15.9Lf, Tp=%15.9Lf, Ta=%15.9Lf, Te=%15.9Lf, Tr=%15.9Lf\n", 0.0L, 0.0L, 0.0L, 0.0L,
+                (long double)r.steps_ms / 1000.0L);
 }
 
 template <typename T, bool CPLX>
@@ -129,12 +171,22 @@ int host_vjsvd(const vj_fnat *m, const vj_fnat *n, T *Gr, const vj_fnat *ldGr, T
   }
   if (*stp)
     VJ_HIP(hipMemcpyAsync(djs, js, (size_t)*stp * N * sizeof(unsigned), hipMemcpyHostToDevice, s));
-  const int rc = plain_svd<T, CPLX>(a, nullptr, s);
+  const bool trace = g_jtrace.load() != 0;
+  FILE *jtr = nullptr;
+  if (trace && !(jtr = fopen((const char *)work, "w")))
+    return CPLX ? -85 : -73; // the reference's fopen failure (dvjsvd.c:73, zvjsvd.c:85)
+  PlainRun run{};
+  const int rc = plain_svd<T, CPLX>(a, trace ? &run : nullptr, s);
   const int L_MG = CPLX ? 92 : 80, L_MV = CPLX ? 104 : 92;
-  if (rc <= VJ_EHIP)
-    return rc;
   // on an input error the reference returns before touching V or eS/fS
   const bool early = (rc == -L_MG) || (rc == -(L_MG + 2)) || (rc == -L_MV) || (rc == (CPLX ? -17 : -13));
+  if (jtr) {
+    if (rc > VJ_EHIP)
+      write_trace(jtr, run, rc, early);
+    (void)fclose(jtr);
+  }
+  if (rc <= VJ_EHIP)
+    return rc;
   if (!early) {
     VJ_HIP(hipMemcpy2DAsync(Gr, (size_t)*ldGr * sizeof(T), a.Gr, M * sizeof(T), M * sizeof(T), N, hipMemcpyDeviceToHost, s));
     if (CPLX)
@@ -202,6 +254,8 @@ int dev_vjsvd(vj_fnat m, vj_fnat n, T *Gr, T *Gi, vj_fnat ldG, T *Vr, T *Vi, vj_
 } // namespace
 
 extern "C" {
+
+void vjh_set_jtrace(int on) { g_jtrace.store(on ? 1 : 0); }
 
 vj_fint dvjsvd_(const vj_fnat *m, const vj_fnat *n, double *G, const vj_fnat *ldG, double *V, const vj_fnat *ldV,
                 double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work,

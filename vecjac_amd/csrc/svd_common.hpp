@@ -45,12 +45,34 @@ template <typename T> struct PlainState {
   unsigned blocks_done;          // last-workgroup detection
   unsigned long long swt;        // transformations in the current sweep
   unsigned long long hist[64];   // transformations per sweep (trace)
+  // trace of the scalings (the reference's JTRACE lines, dvjsvd.c:70-635)
+  T MG0;                         // max |G| on entry
+  int eM0, sR0;                  // its exponent and the rotation headroom
+  unsigned nev;                  // rescalings logged in ev[]
+  struct {
+    unsigned sw, st;             // sweep and step the rescaling precedes
+    int eM, sR, sN;              // exponent of MG, headroom, applied shift
+    T M;                         // MG after the shift
+  } ev[32];
+  unsigned err_sw, err_st;       // sweep and step of a failure
 };
 
 struct PlainRun {
   unsigned sweeps, launched_sweeps;
   int sT, sV;
   unsigned long long hist[64];
+  // trace (the reference's JTRACE output)
+  double MG0, MG_scaled, MG_final;
+  int eM0, sR0, sN0;
+  int zero, status;
+  unsigned nev;
+  struct {
+    unsigned sw, st;
+    int eM, sR, sN;
+    double M;
+  } ev[32];
+  unsigned err_sw, err_st;
+  float steps_ms;                // GPU time of the sweeps (HIP events)
 };
 
 template <typename T, bool CPLX> int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s);

@@ -39,6 +39,9 @@ __global__ void k_setup(const int m, PlainState<T> *st, const int vpreset)
     return;
   const int L_MG = CPLX ? 92 : 80, L_MV = CPLX ? 104 : 92;
   const T MG = (T)__longlong_as_double((long long)st->MG_u);
+  st->MG0 = MG;
+  st->nev = 0u;
+  st->err_sw = st->err_st = 0u;
   const T MV = vpreset ? (T)__longlong_as_double((long long)st->MV_u) : (T)1;
   st->sT = st->sV = 0;
   st->sG_next = st->sV_next = 0;
@@ -62,6 +65,8 @@ __global__ void k_setup(const int m, PlainState<T> *st, const int vpreset)
   const int NRM_EXP = frexp_exp(M_m) - 1;
   const int eM = frexp_exp(MG) - 1;
   const int sN = NRM_EXP - eM - 1;
+  st->eM0 = eM;
+  st->sR0 = (CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R) - eM;
   st->NRM_EXP = NRM_EXP;
   st->MG = ldexp_(MG, sN);
   st->sG_init = sN;
@@ -280,6 +285,8 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   __threadfence();
   st->blocks_done = 0u;
   if (st->status) {
+    st->err_sw = st->sweeps;
+    st->err_st = st_idx;
     st->done = 1;
     return;
   }
@@ -300,6 +307,16 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
       sG = st->NRM_EXP - eM - 1;
       MG = ldexp_(MG, sG);
       st->sT += sG;
+      if (st->nev < 32u) {  // logged against the step it precedes
+        const bool wrap = (st_idx + 1u == stp);
+        auto &e = st->ev[st->nev++];
+        e.sw = st->sweeps + (wrap ? 1u : 0u);
+        e.st = wrap ? 0u : st_idx + 1u;
+        e.eM = eM;
+        e.sR = rot - eM;
+        e.sN = sG;
+        e.M = MG;
+      }
     }
   }
   {
@@ -394,6 +411,8 @@ static StepFn<T, CPLX> pick_step(long long m, int &B_out)
   return nullptr;
 }
 
+template <typename T> static T ldexp_host(T x, int e) { return std::ldexp(x, e); }
+
 template <typename T, bool CPLX>
 int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
 {
@@ -431,7 +450,11 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   VJ_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   int rc = 0;
   unsigned launched = 0;
-  for (unsigned sw = 0; sw < a.swp; ++sw) {
+  hipEvent_t tev[2] = {nullptr, nullptr};
+  if (run && (hipEventCreate(&tev[0]) != hipSuccess || hipEventCreate(&tev[1]) != hipSuccess ||
+              hipEventRecord(tev[0], s) != hipSuccess))
+    rc = set_error(VJ_EHIP, "event creation failed");
+  for (unsigned sw = 0; !rc && sw < a.swp; ++sw) {
     for (unsigned k = 0; k < a.stp; ++k)
       step(a, st, a.js + (size_t)k * a.n, k, tol, gst, s);
     if ((rc = check_launch()))
@@ -456,11 +479,19 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
                        a.Vr, a.Vi, a.ldV, a.eS, a.fS, st, a.swp);
     rc = check_launch();
   }
+  if (!rc && run && hipEventRecord(tev[1], s) != hipSuccess)
+    rc = set_error(VJ_EHIP, "event record failed");
   PlainState<T> h;
   if (!rc) {
     if (hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
       rc = set_error(VJ_EHIP, "state copy failed");
   }
+  float steps_ms = 0.0f;
+  if (!rc && run && hipEventElapsedTime(&steps_ms, tev[0], tev[1]) != hipSuccess)
+    steps_ms = -1.0f;
+  for (int i = 0; i < 2; ++i)
+    if (tev[i])
+      (void)hipEventDestroy(tev[i]);
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   (void)hipHostFree(hf);
@@ -474,6 +505,26 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
     run->launched_sweeps = launched;
     for (int i = 0; i < 64; ++i)
       run->hist[i] = h.hist[i];
+    run->MG0 = (double)h.MG0;
+    run->MG_scaled = (double)ldexp_host(h.MG0, h.sG_init);
+    run->MG_final = (double)h.MG;
+    run->eM0 = h.eM0;
+    run->sR0 = h.sR0;
+    run->sN0 = h.sG_init;
+    run->zero = h.zero;
+    run->status = h.status;
+    run->nev = h.nev;
+    for (unsigned i = 0; i < h.nev && i < 32u; ++i) {
+      run->ev[i].sw = h.ev[i].sw;
+      run->ev[i].st = h.ev[i].st;
+      run->ev[i].eM = h.ev[i].eM;
+      run->ev[i].sR = h.ev[i].sR;
+      run->ev[i].sN = h.ev[i].sN;
+      run->ev[i].M = (double)h.ev[i].M;
+    }
+    run->err_sw = h.err_sw;
+    run->err_st = h.err_st;
+    run->steps_ms = steps_ms;
   }
   if (h.status)
     return h.status;

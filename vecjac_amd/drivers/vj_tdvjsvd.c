@@ -7,6 +7,8 @@
  *   writes  BaseName.S2 / .S4   n singular values (long double), J = 2 / 4;
  *           BaseName.U2 / .U4   U, m x n;  BaseName.V2 / .V4  V, n x n;
  *   prints  "BaseName",J,m,n,seconds,sweeps  (the dvjsvd_ return value).
+ * With the environment variable VJ_JTRACE set, the trace of the reference's
+ * JTRACE build is written to BaseName.J (see vjh_set_jtrace).
  * J is the pjs strategy: 2 (ME) or 4 (MM).  n must be even with n/2 a
  * multiple of 8 and n <= m, as the reference requires.
  */
@@ -54,6 +56,10 @@ int main(int argc, char *argv[])
   if (!iwork)
     return 9;
   *iwork = 0u;
+  if (getenv("VJ_JTRACE")) { /* the trace of the reference's JTRACE build: BaseName.J */
+    (void)snprintf((char *)work, 8u * n * sizeof(double), "%s.%ld", bn, j);
+    vjh_set_jtrace(1);
+  }
   const unsigned swp = 999u;
   const long double t0 = vj_now();
   const vj_fint o = dvjsvd_(&m, &n, G, &ldG, V, &ldV, eS, fS, js, &stp, &swp, work, iwork);

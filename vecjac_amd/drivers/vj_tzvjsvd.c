@@ -59,6 +59,10 @@ int main(int argc, char *argv[])
   if (!iwork)
     return 9;
   *iwork = 0u;
+  if (getenv("VJ_JTRACE")) { /* the trace of the reference's JTRACE build: BaseName.J */
+    (void)snprintf((char *)work, 8u * n * sizeof(double), "%s.%ld", bn, j);
+    vjh_set_jtrace(1);
+  }
   const unsigned swp = 999u;
   t = vj_now();
   const vj_fint o = zvjsvd_(&m, &n, Gr, &ld, Gi, &ld, Vr, &ldV, Vi, &ldV, eS, fS, js, &stp, &swp, work, iwork);
