@@ -33,8 +33,10 @@ typedef uint64_t vj_fnat;  /* fnat: unsigned long long (common.h:185-186) */
  * reference's prototypes), one (re, im) pair of doubles seen from C++ */
 #ifdef __cplusplus
 typedef double vj_zdouble;
+typedef float vj_cfloat;
 #else
 typedef double _Complex vj_zdouble;
+typedef float _Complex vj_cfloat;
 #endif
 
 #ifdef __cplusplus
@@ -60,9 +62,9 @@ vj_fint cbjac2_(const vj_fint *n, const float *a11, const float *a22, const floa
  * (dvjsvd.h:10, dvjsvd.c:40).  On return G holds U (unit columns), V the
  * right singular vectors, and sigma_j = fS[j] * 2^eS[j].  js is the stp x n
  * step table (every row a permutation of 0..n-1, else -9), swp the sweep
- * limit, iwork[0] == 0 means V := I on entry.  work is not used (the
- * reference's JTRACE file name is not supported).  Returns the number of
- * sweeps, or a negative code with the reference's meaning. */
+ * limit, iwork[0] == 0 means V := I on entry.  work holds the trace file
+ * name when the trace is on (vjh_set_jtrace), else it is not used.  Returns
+ * the number of sweeps, or a negative code with the reference's meaning. */
 vj_fint dvjsvd_(const vj_fnat *m, const vj_fnat *n, double *G, const vj_fnat *ldG, double *V, const vj_fnat *ldV,
                 double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work,
                 unsigned *iwork);
@@ -99,6 +101,38 @@ vj_fint zsplit_(const vj_fnat *m, const vj_fnat *n, const vj_zdouble *A, const v
 vj_fint zmerge_(const vj_fnat *m, const vj_fnat *n, const double *Ar, const vj_fnat *ldAr, const double *Ai,
                 const vj_fnat *ldAi, vj_zdouble *A, const vj_fnat *ldA);
 void pjs_free(unsigned *tab);
+
+/* LAPACK xGESVJ-style SVD with a sweep limit; replaces dnssvj_ (dnssvj.f:335,
+ * called as in tdgesvj.c:7), snssvj_ (snssvj.f:321), znssvj_ (znssvj.f:349,
+ * tzgesvj.c:8) and cnssvj_ (cnssvj.f:349).  xGESVJ's arguments plus NSWEEP:
+ * JOBA 'G'/'L'/'U', JOBU 'U'/'C'/'N' (complex also 'F'), JOBV 'V'/'A'/'N'
+ * (complex also 'J'), M >= N >= 0, A (LDA x N, interleaved complex), SVA(N),
+ * V (LDV x N; MV rows for JOBV = 'A'), WORK(LWORK >= max(M+N, 6)) for the
+ * real forms, CWORK(LWORK >= M+N) and RWORK(LRWORK >= max(N, 6)) for the
+ * complex ones (LWORK or LRWORK = -1: workspace query).  Same argument checks
+ * and INFO codes as the reference; INFO = NSWEEP - 1 when NSWEEP sweeps did
+ * not converge.  On exit sigma = WORK(1) * SVA(1:N), non-increasing, A holds
+ * U (first WORK(3) columns normalised; U * SVA for JOBU = 'N'), V the right
+ * singular vectors (or V * (rotations) for 'A'), WORK(2) the number of
+ * nonzero values, WORK(3) those above the underflow threshold, WORK(4) the
+ * sweep index at exit, WORK(5) max |cos| between the columns paired in the
+ * last sweep, WORK(6) max |sin| of its rotations (RWORK for complex).  The
+ * SVD runs on the GPU with this library's parallel Jacobi (strategy pjs id 2),
+ * so sweep counts and the vectors of clustered values are not LAPACK's.
+ * NSWEEP < 1 is rejected (INFO = -14, complex -16; the reference runs no
+ * sweep there); INFO below -16 is a failure of the GPU run (vjh_last_error). */
+void dnssvj_(const char *joba, const char *jobu, const char *jobv, const vj_fint *m, const vj_fint *n, double *A,
+             const vj_fint *lda, double *sva, const vj_fint *mv, double *V, const vj_fint *ldv, double *work,
+             const vj_fint *lwork, const vj_fint *nsweep, vj_fint *info);
+void snssvj_(const char *joba, const char *jobu, const char *jobv, const vj_fint *m, const vj_fint *n, float *A,
+             const vj_fint *lda, float *sva, const vj_fint *mv, float *V, const vj_fint *ldv, float *work,
+             const vj_fint *lwork, const vj_fint *nsweep, vj_fint *info);
+void znssvj_(const char *joba, const char *jobu, const char *jobv, const vj_fint *m, const vj_fint *n, vj_zdouble *A,
+             const vj_fint *lda, double *sva, const vj_fint *mv, vj_zdouble *V, const vj_fint *ldv, vj_zdouble *cwork,
+             const vj_fint *lwork, double *rwork, const vj_fint *lrwork, const vj_fint *nsweep, vj_fint *info);
+void cnssvj_(const char *joba, const char *jobu, const char *jobv, const vj_fint *m, const vj_fint *n, vj_cfloat *A,
+             const vj_fint *lda, float *sva, const vj_fint *mv, vj_cfloat *V, const vj_fint *ldv, vj_cfloat *cwork,
+             const vj_fint *lwork, float *rwork, const vj_fint *lrwork, const vj_fint *nsweep, vj_fint *info);
 
 /* ---- 2. device-memory extensions ---------------------------------------- */
 

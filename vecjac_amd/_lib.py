@@ -77,6 +77,13 @@ def _declare_svd(L) -> None:
     dcv = [_fnat, _fnat, _pd, _pd, _fnat, _pd, _pd, _fnat, _pd, _pd, _pu, U, U, U, C.c_void_p, _pu]
     for name, args in (("vjh_dvjsvd", dsv), ("vjh_svjsvd", dsv), ("vjh_zvjsvd", dcv), ("vjh_cvjsvd", dcv)):
         _decl(L, name, _fint, args)
+    # LAPACK xGESVJ-style drivers (subroutines: INFO is the last argument)
+    F = P(_fint)
+    ch = [C.c_char_p] * 3
+    rn = ch + [F, F, _pd, F, _pd, F, _pd, F, _pd, F, F, F]
+    cn = ch + [F, F, _pd, F, _pd, F, _pd, F, _pd, F, _pd, F, F, F]
+    for name, args in (("dnssvj_", rn), ("snssvj_", rn), ("znssvj_", cn), ("cnssvj_", cn)):
+        _decl(L, name, None, args)
 
 
 def last_error() -> str:

@@ -20,6 +20,11 @@ template <typename T> struct PlainArgs {
   const unsigned *js; // device step table, stp rows of n entries
   unsigned stp, swp;
   int vpreset;        // iwork[0] != 0: V holds a starting matrix
+  // options of the LAPACK-style drivers (zero: the reference's behaviour)
+  long long nv;       // rows of V (0: n); a preset nv x n V has the rotations applied
+  T tol;              // convergence threshold (0: sqrt(m) eps/2)
+  int raw;            // no epilogue: G, V and eS stay in their scaled working form
+  int stats;          // record max |cos| and max |sin| of the pairs of every sweep
 };
 
 // The first two words are copied to the host once per sweep.
@@ -55,6 +60,10 @@ template <typename T> struct PlainState {
     T M;                         // MG after the shift
   } ev[32];
   unsigned err_sw, err_st;       // sweep and step of a failure
+  // pair statistics (stats != 0): maxima of |cos(x, y)| over the pairs
+  // examined and of |sin| over the rotations applied, as bit patterns of
+  // non-negative doubles; folded into last_* at the end of every sweep
+  unsigned long long cos_u, sin_u, last_cos_u, last_sin_u;
 };
 
 struct PlainRun {
@@ -73,6 +82,8 @@ struct PlainRun {
   } ev[32];
   unsigned err_sw, err_st;
   float steps_ms;                // GPU time of the sweeps (HIP events)
+  double last_cos, last_sin;     // pair statistics of the last sweep run
+  int sG_pending, sV_pending;    // shifts counted in sT / sV but not yet applied
 };
 
 template <typename T, bool CPLX> int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s);

@@ -334,6 +334,7 @@ template <typename T> struct PairDec {
   int trans;         // 1 if transformed (counted in the sweep total)
   int err;           // 0, or the reference's negative line code
   T c, at, sat;      // cosine and (complex) tangent of the rotation
+  T cosv;            // |a21|: |cos| of the angle between the two columns
 };
 
 template <typename T, bool CPLX> struct PairOut {
@@ -470,7 +471,7 @@ pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[
 template <typename T, bool CPLX>
 __device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T tol, const T gst)
 {
-  PairDec<T> d{1, 0, 0, (T)1, (T)0, (T)0};
+  PairDec<T> d{1, 0, 0, (T)1, (T)0, (T)0, (T)0};
   if (!isfinite_(P.a21r) || !isfinite_(P.a21i)) {
     d.err = CPLX ? -345 : -320;
     return d;
@@ -485,6 +486,7 @@ __device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T 
   }
   else
     aabs = vabs(P.a21r);
+  d.cosv = aabs;
   if (tol <= aabs) {
     d.trans = 1;
     const T e1 = P.ex, e2 = P.ey, f1 = P.fx, f2 = P.fy;
@@ -764,11 +766,11 @@ template <typename T, bool CPLX, int B, int E>
 __device__ __forceinline__ PairOut<T, CPLX>
 pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
           T *__restrict__ Vrp, T *__restrict__ Vrq, T *__restrict__ Vip, T *__restrict__ Viq,
-          T &eSp, T &fSp, T &eSq, T &fSq, const Step<T> &S, double *lds, const int lane)
+          T &eSp, T &fSp, T &eSq, T &fSq, const Step<T> &S, double *lds, const int lane, PairDec<T> &D)
 {
   (void)m;
   const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, lds);
-  const PairDec<T> D = pair_decide<T, CPLX>(P, S.tol, S.gst);
+  D = pair_decide<T, CPLX>(P, S.tol, S.gst);
   return pair_apply<T, CPLX, B, E>(n, xr, yr, xi, yi, Vrp, Vrq, Vip, Viq, P, D, S.vtrack, eSp, fSp, eSq, fSq, lds, lane);
 }
 

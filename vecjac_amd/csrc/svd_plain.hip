@@ -10,6 +10,8 @@
 // host round trip; once the flag is set, later launches return at once.
 #include "svd_common.hpp"
 
+#include <string.h>
+
 namespace vj {
 
 // ------------------------------------------------------------ setup kernels
@@ -183,8 +185,9 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
                                              const long long ldV, T *__restrict__ eS, T *__restrict__ fS,
                                              const unsigned *__restrict__ row, const unsigned st_idx,
                                              const unsigned stp, const unsigned swp, const T tol, const T gst,
-                                             const int vtrack, PlainState<T> *__restrict__ st)
+                                             const int vtrack, const int stats, PlainState<T> *__restrict__ st)
 {
+  // n is the length of the V columns (the table row has gridDim.x pairs)
   __shared__ double lds[8 * (B / 64) + 2];
   __shared__ int last;
   if (*(volatile int *)&st->done)
@@ -215,10 +218,11 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   }
   T eSp, fSp, eSq, fSq;
   PairOut<T, CPLX> o;
+  PairDec<T> D;
   if constexpr (!STREAM) {
     T xr[E], yr[E], xi[E], yi[E];
     load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, S.sG, lane, xr, yr, xi, yi);
-    o = pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, lds, lane);
+    o = pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, lds, lane, D);
     if (!o.err && ((o.code != 1) || S.sG))
       store_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, lane, xr, yr, xi, yi);
   }
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     }
     red_dd4<A, B>(sx, sy, dr, di, lds);
     const PairScal<T> P = pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
-    const PairDec<T> D = pair_decide<T, CPLX>(P, S.tol, S.gst);
+    D = pair_decide<T, CPLX>(P, S.tol, S.gst);
     T tG = (T)0;
     for (int c = 0; c < nc; ++c) {
       load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
@@ -273,6 +277,15 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
       atomicMax(&st->nMV_u, (unsigned long long)__double_as_longlong((double)o.tV));
       if (o.trans)
         atomicAdd(&st->swt, 1ull);
+      if (stats) {
+        // |sin| of the rotation: c |tan|, the complex tangent being at + i sat
+        T ta = vabs(D.at);
+        if constexpr (CPLX)
+          ta = sqrt_(fma_(D.sat, D.sat, ta * ta));
+        const T sn = D.trans ? D.c * ta : (T)0;
+        atomicMax(&st->cos_u, (unsigned long long)__double_as_longlong((double)D.cosv));
+        atomicMax(&st->sin_u, (unsigned long long)__double_as_longlong((double)sn));
+      }
     }
     __threadfence();
     last = (atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1u);
@@ -332,6 +345,11 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   st->sG_next = sG;
   st->sV_next = sV;
   if (st_idx == stp - 1u) {
+    if (stats) {
+      st->last_cos_u = st->cos_u;
+      st->last_sin_u = st->sin_u;
+      st->cos_u = st->sin_u = 0ull;
+    }
     const unsigned long long swt = st->swt;
     if (st->sweeps < 64u)
       st->hist[st->sweeps] = swt;
@@ -381,8 +399,9 @@ __global__ __launch_bounds__(256) void k_epilogue(const int m, const int n, T *_
 template <typename T, bool CPLX, int B, int E, bool STREAM = false>
 static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, const unsigned *row, unsigned s, const T tol, const T gst, hipStream_t strm)
 {
-  hipLaunchKernelGGL((k_step<T, CPLX, B, E, STREAM>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m, (int)a.n, a.Gr, a.Gi, a.ldG,
-                     a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp, tol, gst, a.vpreset, st);
+  hipLaunchKernelGGL((k_step<T, CPLX, B, E, STREAM>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m,
+                     (int)(a.nv ? a.nv : a.n), a.Gr, a.Gi, a.ldG, a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp,
+                     tol, gst, a.vpreset, a.stats, st);
   return 0;
 }
 
@@ -412,6 +431,12 @@ static StepFn<T, CPLX> pick_step(long long m, int &B_out)
 }
 
 template <typename T> static T ldexp_host(T x, int e) { return std::ldexp(x, e); }
+static double bits_to_double(unsigned long long u)
+{
+  double d;
+  memcpy(&d, &u, sizeof(d));
+  return d;
+}
 
 template <typename T, bool CPLX>
 int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
@@ -429,17 +454,18 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gr, a.ldG, &st->MG_u);
   if (CPLX)
     hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gi, a.ldG, &st->MG_u);
+  const long long nv = a.nv ? a.nv : a.n;
   if (a.vpreset) {
-    hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.n, (int)a.n, a.Vr, a.ldV, &st->MV_u);
+    hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)nv, (int)a.n, a.Vr, a.ldV, &st->MV_u);
     if (CPLX)
-      hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.n, (int)a.n, a.Vi, a.ldV, &st->MV_u);
+      hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)nv, (int)a.n, a.Vi, a.ldV, &st->MV_u);
   }
   hipLaunchKernelGGL((k_setup<T, CPLX>), dim3(1), dim3(64), 0, s, (int)a.m, st, a.vpreset);
-  hipLaunchKernelGGL(k_init_cols<T>, dim3((unsigned)a.n), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gr, a.Gi, a.ldG,
+  hipLaunchKernelGGL(k_init_cols<T>, dim3((unsigned)a.n), dim3(256), 0, s, (int)a.m, (int)nv, a.Gr, a.Gi, a.ldG,
                      a.Vr, a.Vi, a.ldV, a.eS, a.fS, st, a.vpreset);
   if (int rc = check_launch())
     return rc;
-  const T tol = host_tol<T>(a.m);
+  const T tol = (a.tol > (T)0) ? a.tol : host_tol<T>(a.m);
   const T gst = host_gst<T>(tol);
   // Sweeps are enqueued one ahead of the convergence check: the host only
   // reads the device flag of sweep k-1 after sweep k has been queued.
@@ -474,8 +500,8 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
         break;
     }
   }
-  if (!rc) {
-    hipLaunchKernelGGL(k_epilogue<T>, dim3((unsigned)a.n), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gr, a.Gi, a.ldG,
+  if (!rc && !a.raw) {
+    hipLaunchKernelGGL(k_epilogue<T>, dim3((unsigned)a.n), dim3(256), 0, s, (int)a.m, (int)nv, a.Gr, a.Gi, a.ldG,
                        a.Vr, a.Vi, a.ldV, a.eS, a.fS, st, a.swp);
     rc = check_launch();
   }
@@ -525,6 +551,10 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
     run->err_sw = h.err_sw;
     run->err_st = h.err_st;
     run->steps_ms = steps_ms;
+    run->last_cos = bits_to_double(h.last_cos_u);
+    run->last_sin = bits_to_double(h.last_sin_u);
+    run->sG_pending = h.sG_next;
+    run->sV_pending = h.sV_next;
   }
   if (h.status)
     return h.status;

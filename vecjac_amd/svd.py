@@ -5,6 +5,8 @@ z/s/c twins take numpy arrays laid out exactly as the reference's arguments
 (column-major, G[j * ldG + i]); they call the drop-in C entry points of
 libvecjac_hip.so.  `device_vjsvd` runs the same driver on torch CUDA
 tensors.  `pjs(id, n)` returns the strategy table as a uint32 array.
+`nssvj(kind, joba, jobu, jobv, m, n, A, lda, sva, mv, V, ldv, ...)` calls the
+LAPACK xGESVJ-style drivers ?nssvj_ with LAPACK's argument list.
 """
 from __future__ import annotations
 
@@ -113,3 +115,71 @@ def device_vjsvd(kind: str, G, V, eS, fS, js_dev, stp: int, swp: int = 999, Gi=N
                fS.data_ptr(), js_dev.data_ptr(), stp, swp, int(vpreset), s, stats)
     _lib.check(rc, f"vjh_{kind}vjsvd")
     return rc, (stats[0], stats[1], list(stats[2:2 + 64]))
+
+
+def _i64(x):
+    return C.byref(C.c_int64(int(x)))
+
+
+def nssvj(kind: str, joba: str, jobu: str, jobv: str, m, n, A, lda, sva, mv, V, ldv, work, lwork, nsweep,
+          rwork=None, lrwork=None) -> int:
+    """Call ?nssvj_ (dnssvj.f:335, znssvj.f:349) with LAPACK's argument list
+    and return INFO.  Arrays are numpy: A and V column-major with leading
+    dimensions lda / ldv (complex128 / complex64 for z / c), work the real
+    WORK (d, s) or the complex CWORK (z, c) with rwork the real RWORK."""
+    L = _lib.lib()
+    f = getattr(L, f"{kind}nssvj_")
+    want = {"d": np.float64, "s": np.float32, "z": np.complex128, "c": np.complex64}[kind]
+    if A.dtype != want:
+        raise TypeError("dtype does not match the routine")
+    info = C.c_int64(0)
+    ptr = lambda x: None if x is None else x.ctypes.data  # noqa: E731
+    args = [joba.encode(), jobu.encode(), jobv.encode(), _i64(m), _i64(n), ptr(A), _i64(lda), ptr(sva), _i64(mv),
+            ptr(V), _i64(ldv), ptr(work), _i64(lwork)]
+    if kind in "zc":
+        args += [ptr(rwork), _i64(lrwork)]
+    args += [_i64(nsweep), C.byref(info)]
+    f(*args)
+    return _lib.check(info.value, f"{kind}nssvj_")
+
+
+def gesvj(A: np.ndarray, joba: str = "G", jobu: str = "U", jobv: str = "V", nsweep: int = 30, ctol=None,
+          V0: np.ndarray = None):
+    """xGESVJ-style SVD of the m x n matrix A (m >= n) through ?nssvj_.
+    Returns (sva, U, V, stat, info) with sigma = stat[0] * sva; U is A on
+    exit (m x n), V the n x n right singular vectors (JOBV 'V'), V0 times the
+    rotations (JOBV 'A', V0 of shape (mv, n)), or None (JOBV 'N')."""
+    A = np.asarray(A)
+    kind = {np.dtype(np.float64): "d", np.dtype(np.float32): "s", np.dtype(np.complex128): "z",
+            np.dtype(np.complex64): "c"}[A.dtype]
+    real_t = np.float64 if kind in "dz" else np.float32
+    m, n = A.shape
+    Af = np.asfortranarray(A.copy())
+    sva = np.zeros(max(n, 1), real_t)
+    if jobv.upper() == "A":
+        mv = V0.shape[0]
+        Vf = np.asfortranarray(np.asarray(V0, dtype=A.dtype).copy())
+        ldv = max(mv, 1)
+    else:
+        mv = 0
+        Vf = np.asfortranarray(np.zeros((max(n, 1), max(n, 1)), A.dtype))
+        ldv = max(n, 1)
+    if kind in "ds":
+        lwork = max(m + n, 6)
+        work = np.zeros(lwork, real_t)
+        if ctol is not None:
+            work[0] = ctol
+        info = nssvj(kind, joba, jobu, jobv, m, n, Af, max(m, 1), sva, mv, Vf, ldv, work, lwork, nsweep)
+        stat = work[:6].copy()
+    else:
+        lwork = max(m + n, 1)
+        cwork = np.zeros(lwork, A.dtype)
+        lrwork = max(n, 6)
+        rwork = np.zeros(lrwork, real_t)
+        if ctol is not None:
+            rwork[0] = ctol
+        info = nssvj(kind, joba, jobu, jobv, m, n, Af, max(m, 1), sva, mv, Vf, ldv, cwork, lwork, nsweep,
+                     rwork, lrwork)
+        stat = rwork[:6].copy()
+    V = None if jobv.upper() == "N" else Vf[:mv, :n] if jobv.upper() == "A" else Vf[:n, :n]
+    return sva[:n], Af, V, stat, info
