@@ -21,16 +21,22 @@ struct BPlan {
   int kind = 0;            // 0 d, 1 z, 2 s, 3 c
   int nb = 0;
   int max_m = 0;
+  int nlong = 0;           // matrices with m > short_rows: first in the order, one launch
+  int max_m_long = 0, max_m_short = 0;
   int waves = 4;
   long long totG = 0;      // elements of the packed G (max over b of offG + m n)
   unsigned swp = 999u;
   int dev = 0;
   long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each)
   int *d_order = nullptr;
-  unsigned *d_queue = nullptr;
+  unsigned *d_queue = nullptr; // work counters of the two launches
 };
 
-__global__ void k_zero_u32(unsigned *p) { *p = 0u; }
+__global__ void k_zero_u32(unsigned *p)
+{
+  p[0] = 0u;
+  p[1] = 0u;
+}
 
 int pick_waves(const std::vector<long long> &n)
 {
@@ -71,11 +77,19 @@ int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, v
   a.fS = (T *)fS;
   a.info = (long long *)info;
   a.transforms = (unsigned long long *)transforms;
-  a.queue = P->d_queue;
   a.swp = P->swp;
   // a fill kernel, not hipMemsetAsync, so that the run can be graph-captured
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, P->d_queue);
-  return batched_svd<T, CPLX>(a, P->waves, P->max_m, 0, s);
+  // the long columns first (the costly matrices start first), then the
+  // short ones in the lane-group form, each launch with its own queue
+  a.nb = P->nlong;
+  a.queue = P->d_queue;
+  if (int rc = batched_svd<T, CPLX>(a, P->waves, P->max_m_long, 0, false, s))
+    return rc;
+  a.nb = P->nb - P->nlong;
+  a.order = P->d_order + P->nlong;
+  a.queue = P->d_queue + 1;
+  return batched_svd<T, CPLX>(a, P->waves, P->max_m_short, 0, true, s);
 }
 
 } // namespace
@@ -114,12 +128,25 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   P->swp = swp;
   P->waves = cap_waves(kind, max_m, (waves > 0) ? waves : (nb ? pick_waves(hn) : 4));
   (void)hipGetDevice(&P->dev);
-  // processing order: decreasing m n^2 (longest first)
+  // processing order: the long-column class, then the short one, each by
+  // decreasing m n^2 (longest first)
+  const long long sr = short_rows((kind & 1) != 0);
   std::vector<int> order((size_t)nb);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const bool lx = hm[x] > sr, ly = hm[y] > sr;
+    if (lx != ly)
+      return lx;
     return hm[x] * hn[x] * hn[x] > hm[y] * hn[y] * hn[y];
   });
+  for (vj_fint b = 0; b < nb; ++b) {
+    if (hm[b] > sr) {
+      ++P->nlong;
+      P->max_m_long = std::max(P->max_m_long, (int)hm[b]);
+    }
+    else
+      P->max_m_short = std::max(P->max_m_short, (int)hm[b]);
+  }
   std::vector<long long> meta;
   meta.reserve((size_t)(5 * nb));
   meta.insert(meta.end(), hm.begin(), hm.end());
@@ -130,7 +157,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_meta, std::max<size_t>(1, meta.size()) * sizeof(long long));
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_order, std::max<size_t>(1, order.size()) * sizeof(int));
-  if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, sizeof(unsigned));
+  if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, 2 * sizeof(unsigned));
   if (e == hipSuccess && !meta.empty()) e = hipMemcpy(P->d_meta, meta.data(), meta.size() * sizeof(long long), hipMemcpyHostToDevice);
   if (e == hipSuccess && !order.empty()) e = hipMemcpy(P->d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) {

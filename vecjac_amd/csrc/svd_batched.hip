@@ -15,6 +15,12 @@
 // is folded through double-buffered LDS slots, so every thread takes the
 // same rescaling and convergence decisions; no host round trip.  The
 // register footprint (64-row chunks per lane) is chosen per matrix.
+//
+// Short columns (m <= 128 real, 64 complex) go to a second launch of the
+// same kernel in its lane-group form (group_step): four pairs per wave, one
+// per row of 16 lanes, so that the reductions, the decisions and their
+// scalar work are shared by four pairs instead of two.  The two forms are
+// separate launches because one body holding both exceeds 256 registers.
 #include "svd_common.hpp"
 #include "svd_batched.hpp"
 #include "pjs.hpp"
@@ -392,6 +398,453 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
   }
 }
 
+// ------------------------------------------------------- lane-group steps
+// A wave works on P = 64 / LP pairs at once, one per group of LP lanes
+// (LP = 16: a DPP row; 32: two rows; 64: the whole wave).  Lane t of a group
+// holds rows t + LP j (j < E) of the pair's G columns and of its V columns
+// (n <= m rows, so E chunks cover them too).  The reductions run inside the
+// groups (DPP within a row, then permlane16 / permlane32 swaps) and leave
+// the group totals in every lane of the group, so every lane decides its own
+// group's pair (the decisions differ between groups, not within one) and
+// the update needs no broadcast.  A permuting update (codes -1, -2) swaps
+// the destination columns instead of the registers.
+template <typename T> __device__ __forceinline__ T group_max(T v, const int LP)
+{
+  v = row_max(v);
+  if (LP >= 32) {
+    T a = v, b = v;
+    swap_<false>(a, b); // a: rows (v0, v0, v2, v2), b: rows (v1, v1, v3, v3)
+    v = xmax(a, b);
+  }
+  if (LP >= 64) {
+    T a = v, b = v;
+    swap_<true>(a, b);
+    v = xmax(a, b);
+  }
+  return v;
+}
+template <typename A> __device__ __forceinline__ DD<A> group_sum(DD<A> v, const int LP)
+{
+  v = row_sum(v);
+  if (LP >= 32) {
+    DD<A> a = v, b = v;
+    swap_<false>(a.s, b.s);
+    swap_<false>(a.c, b.c);
+    v = dd_add(a, b); // dd_add is commutative bit for bit: every lane agrees
+  }
+  if (LP >= 64) {
+    DD<A> a = v, b = v;
+    swap_<true>(a.s, b.s);
+    swap_<true>(a.c, b.c);
+    v = dd_add(a, b);
+  }
+  return v;
+}
+
+// Columns p, q of a rows-row matrix (ld rows) into registers, row t + LP j;
+// all E chunks are loaded (rows past the end read the last row), so the
+// number of loads in flight is fixed.
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ void load_cols(const T *Ar, const T *Ai, const int rows, const unsigned p, const unsigned q,
+                                          const int t, const int LP, T (&x)[E], T (&y)[E], T (&xi)[E], T (&yi)[E])
+{
+  const long long op = (long long)p * rows, oq = (long long)q * rows;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int i = t + LP * j;
+    const int ic = (i < rows) ? i : (rows - 1);
+    x[j] = Ar[op + ic];
+    y[j] = Ar[oq + ic];
+    if constexpr (CPLX) {
+      xi[j] = Ai[op + ic];
+      yi[j] = Ai[oq + ic];
+    }
+    else
+      xi[j] = yi[j] = (T)0;
+  }
+}
+
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ void store_cols(T *Ar, T *Ai, const int rows, const int nE, const unsigned p, const unsigned q,
+                                           const int t, const int LP, const T (&x)[E], const T (&y)[E], const T (&xi)[E],
+                                           const T (&yi)[E])
+{
+  const long long op = (long long)p * rows, oq = (long long)q * rows;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
+    const int i = t + LP * j;
+    if (i < rows) {
+      Ar[op + i] = x[j];
+      Ar[oq + i] = y[j];
+      if constexpr (CPLX) {
+        Ai[op + i] = xi[j];
+        Ai[oq + i] = yi[j];
+      }
+    }
+  }
+}
+
+// Real rotation (x, y) := ((x + t y) c, (y - t x) c) of E chunks, the
+// scaling skipped where c == 1 (the product by one is exact); returns the
+// lane's max |updated element|.
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ T grp_rot_real(T (&x)[E], T (&y)[E], T (&xi)[E], T (&yi)[E], const T c, const T at,
+                                          const bool c1, const int nE)
+{
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
+    const T u = fma_(y[j], at, x[j]), v = fma_(-x[j], at, y[j]);
+    x[j] = u;
+    y[j] = v;
+    if constexpr (CPLX) {
+      const T u2 = fma_(yi[j], at, xi[j]), v2 = fma_(-xi[j], at, yi[j]);
+      xi[j] = u2;
+      yi[j] = v2;
+    }
+  }
+  if (!c1) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      if (j >= nE)
+        break;
+      x[j] = x[j] * c;
+      y[j] = y[j] * c;
+      if constexpr (CPLX) {
+        xi[j] = xi[j] * c;
+        yi[j] = yi[j] * c;
+      }
+    }
+  }
+  T tG = (T)0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
+    tG = max1p(tG, max1p(vabs(x[j]), vabs(y[j])));
+    if constexpr (CPLX)
+      tG = max1p(tG, max1p(vabs(xi[j]), vabs(yi[j])));
+  }
+  return tG;
+}
+
+// Complex rotation with a complex tangent at + i sat (zjrot_, zjrot.c:36-60).
+template <typename T, int E>
+__device__ __forceinline__ T grp_rot_cplx(T (&x)[E], T (&y)[E], T (&xi)[E], T (&yi)[E], const T c, const T at,
+                                          const T sat, const bool c1, const int nE)
+{
+  const T cna = -at;
+  T tG = (T)0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
+    const T x_r = x[j], x_i = xi[j], y_r = y[j], y_i = yi[j];
+    T ar = fma_(y_r, at, fma_(-y_i, sat, x_r));
+    T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
+    T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
+    T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
+    if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+    tG = xmax(tG, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
+    x[j] = ar; xi[j] = ai; y[j] = br; yi[j] = bi;
+  }
+  return tG;
+}
+
+// Gram-Schmidt correction (dgsscl_/zgsscl_, dgsscl.c:5-85, zgsscl.c:5-119)
+// of pair_apply, per lane; for code -3 the result lands in x's place and y
+// takes the old x... as there: x := y, y := the corrected column.
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ T grp_gs(T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E], const PairScal<T> &P,
+                                    const bool perm, const int nE)
+{
+  const T ex = P.ex, fx = P.fx, ey = P.ey, fy = P.fy;
+  const T x_e = -ex, y_e = -ey;
+  T tG = (T)0;
+  if (!perm) {
+    const T fy_x = fy / fx;
+    const T tfr = -P.a21r * fy_x;
+    const T tfi = P.a21i * fy_x;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      if (j >= nE)
+        break;
+      const T xs = scalef_fin(xr[j], x_e);
+      const T ys = scalef_fin(yr[j], y_e);
+      if constexpr (CPLX) {
+        const T xsi = scalef_fin(xi[j], x_e), ysi = scalef_fin(yi[j], y_e);
+        const T nr = fma_(tfr, xs, fma_(-tfi, xsi, ys));
+        const T ni = fma_(tfr, xsi, fma_(tfi, xs, ysi));
+        yr[j] = scalef_fin(nr, ey);
+        yi[j] = scalef_fin(ni, ey);
+        tG = xmax(tG, xmax(vabs(yr[j]), vabs(yi[j])));
+      }
+      else {
+        yr[j] = scalef_fin(fma_(tfr, xs, ys), ey);
+        tG = max1p(tG, vabs(yr[j]));
+      }
+    }
+  }
+  else {
+    const T fx_y = fx / fy;
+    const T tfr = -P.a21r * fx_y;
+    const T tfi = -P.a21i * fx_y;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      if (j >= nE)
+        break;
+      const T xs = scalef_fin(xr[j], x_e);
+      const T ys = scalef_fin(yr[j], y_e);
+      if constexpr (CPLX) {
+        const T xsi = scalef_fin(xi[j], x_e), ysi = scalef_fin(yi[j], y_e);
+        const T nr = fma_(tfr, ys, fma_(-tfi, ysi, xs));
+        const T ni = fma_(tfr, ysi, fma_(tfi, ys, xsi));
+        xr[j] = yr[j];
+        xi[j] = yi[j];
+        yr[j] = scalef_fin(nr, ex);
+        yi[j] = scalef_fin(ni, ex);
+        tG = xmax(tG, xmax(vabs(yr[j]), vabs(yi[j])));
+      }
+      else {
+        xr[j] = yr[j];
+        yr[j] = scalef_fin(fma_(tfr, ys, xs), ex);
+        tG = max1p(tG, vabs(yr[j]));
+      }
+    }
+  }
+  return tG;
+}
+
+// Registers the prefetched (double-buffered) form may use: the G and V
+// pairs of one group step are 4E (complex 8E) elements per lane.
+template <typename T, bool CPLX, int E> struct GroupShape {
+  static constexpr int words = 4 * E * (CPLX ? 2 : 1) * (int)(sizeof(T) / 4);
+  static constexpr bool PF = (words <= 32);
+};
+
+// One step of the strategy for the pairs of one wave (pairs wave, wave + W,
+// wave + 2W, ... of the modulus table of width n, pjs id 4), P pairs at a
+// time in lane groups: load (the next P prefetched when PF), reduce, decide
+// per lane, update in registers, store.  Accumulates this wave's per-lane
+// max of the updated G elements, its transformation count and its first
+// error.
+template <typename T, bool CPLX, int E, int W, bool PF>
+__device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
+                                           const int lg, const unsigned st, const int wave, const int lane,
+                                           const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr)
+{
+  typedef typename Acc<T>::A A;
+  const int LP = 1 << lg, P = 64 >> lg; // lanes per group, groups per wave
+  const int npairs = n >> 1;
+  if (wave >= npairs)
+    return;
+  const int cnt = (npairs - wave + W - 1) / W; // pairs of this wave
+  const int nit = (cnt + P - 1) >> (6 - lg);
+  const int g = lane >> lg, t = lane & (LP - 1);
+  const int nE = (m + LP - 1) >> lg, nEV = (n + LP - 1) >> lg;
+  // pair of this group in iteration it (past the end: the wave's last pair
+  // again, computed and not stored)
+  auto pair_of = [&](const int it, unsigned &pp, unsigned &qq) -> bool {
+    int i = it * P + g;
+    const bool v = (i < cnt);
+    i = v ? i : (cnt - 1);
+    mod_pair((unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
+    return v;
+  };
+  T xr[E], yr[E], xi[E], yi[E], vx[E], vy[E], vxi[E], vyi[E];
+  unsigned p, q;
+  bool valid;
+  if constexpr (PF) {
+    valid = pair_of(0, p, q);
+    load_cols<T, CPLX, E>(Vr, Vi, n, p, q, t, LP, vx, vy, vxi, vyi);
+    load_cols<T, CPLX, E>(Gr, Gi, m, p, q, t, LP, xr, yr, xi, yi);
+  }
+  for (int it = 0; it < nit; ++it) {
+    unsigned pn, qn;
+    bool validn = false;
+    T nxr[E], nyr[E], nxi[E], nyi[E], nvx[E], nvy[E], nvxi[E], nvyi[E];
+    if constexpr (PF) {
+      validn = pair_of((it + 1 < nit) ? (it + 1) : it, pn, qn);
+      load_cols<T, CPLX, E>(Vr, Vi, n, pn, qn, t, LP, nvx, nvy, nvxi, nvyi);
+      load_cols<T, CPLX, E>(Gr, Gi, m, pn, qn, t, LP, nxr, nyr, nxi, nyi);
+    }
+    else {
+      valid = pair_of(it, p, q);
+      load_cols<T, CPLX, E>(Vr, Vi, n, p, q, t, LP, vx, vy, vxi, vyi);
+      load_cols<T, CPLX, E>(Gr, Gi, m, p, q, t, LP, xr, yr, xi, yi);
+    }
+    // ---- rows past m to zero, pending rescalings
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      if (j >= nE)
+        break;
+      const bool in = (t + LP * j) < m;
+      xr[j] = in ? xr[j] : (T)0;
+      yr[j] = in ? yr[j] : (T)0;
+      if constexpr (CPLX) {
+        xi[j] = in ? xi[j] : (T)0;
+        yi[j] = in ? yi[j] : (T)0;
+      }
+      if (S.sG) {
+        xr[j] = ldexp_(xr[j], S.sG);
+        yr[j] = ldexp_(yr[j], S.sG);
+        if constexpr (CPLX) {
+          xi[j] = ldexp_(xi[j], S.sG);
+          yi[j] = ldexp_(yi[j], S.sG);
+        }
+      }
+    }
+    if (S.sV) {
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        vx[j] = ldexp_(vx[j], S.sV);
+        vy[j] = ldexp_(vy[j], S.sV);
+        vxi[j] = ldexp_(vxi[j], S.sV);
+        vyi[j] = ldexp_(vyi[j], S.sV);
+      }
+    }
+    // ---- reduce within the groups
+    T mx, my;
+    pair_maxima<T, CPLX, E>(xr, yr, xi, yi, nE, mx, my);
+    mx = group_max(mx, LP);
+    my = group_max(my, LP);
+    const int kx = max_exp(mx), ky = max_exp(my);
+    DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
+    pair_sums<T, CPLX, E>(xr, yr, xi, yi, nE, kx, ky, sx, sy, dr, di);
+    sx = group_sum(sx, LP);
+    sy = group_sum(sy, LP);
+    dr = group_sum(dr, LP);
+    if constexpr (CPLX)
+      di = group_sum(di, LP);
+    // ---- every lane decides its group's pair
+    const PairScal<T> Pd = pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
+    const PairDec<T> D = pair_decide<T, CPLX>(Pd, S.tol, S.gst);
+    // ---- update in registers
+    const int code = D.code;
+    const bool ok = (D.err == 0);
+    const bool two = ok && ((code == 2) || (code == -2));
+    const bool at0 = (D.at == (T)0) && (!CPLX || (D.sat == (T)0));
+    const bool rotr = two && !at0 && (!CPLX || (D.sat == (T)0));
+    const bool rotc = CPLX && two && (D.sat != (T)0);
+    const bool gs = ok && ((code == 3) || (code == -3));
+    const bool c1 = (D.c == (T)1);
+    // destination columns exchanged: a swap (-1), or a rotation or a unit
+    // swap with the permutation (-2)
+    const bool dsw = ok && ((code == -1) || ((code == -2) && (!at0 || c1)));
+    T tG = (T)0;
+    if (rotr) {
+      tG = grp_rot_real<T, CPLX, E>(xr, yr, xi, yi, D.c, D.at, c1, nE);
+      grp_rot_real<T, CPLX, E>(vx, vy, vxi, vyi, D.c, D.at, c1, nEV);
+    }
+    if constexpr (CPLX) {
+      if (rotc) {
+        tG = grp_rot_cplx<T, E>(xr, yr, xi, yi, D.c, D.at, D.sat, c1, nE);
+        grp_rot_cplx<T, E>(vx, vy, vxi, vyi, D.c, D.at, D.sat, c1, nEV);
+      }
+    }
+    if (gs) {
+      const bool perm = (code < 0);
+      tG = grp_gs<T, CPLX, E>(xr, yr, xi, yi, Pd, perm, nE);
+      if (perm) {
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          T u = vx[j]; vx[j] = vy[j]; vy[j] = u;
+          if constexpr (CPLX) { u = vxi[j]; vxi[j] = vyi[j]; vyi[j] = u; }
+        }
+      }
+    }
+    // an update that overflowed anywhere in the group leaves G unstored
+    const unsigned long long gmask = (lg == 6) ? ~0ull : (((1ull << LP) - 1ull) << (g << lg));
+    int err = D.err;
+    if (!err && ((__ballot(!isfinite_(tG)) & gmask) != 0ull))
+      err = CPLX ? -630 : -580;
+    const unsigned dp = dsw ? q : p, dq = dsw ? p : q;
+    if (valid && !err && ((code != 1) || S.sG))
+      store_cols<T, CPLX, E>(Gr, Gi, m, nE, dp, dq, t, LP, xr, yr, xi, yi);
+    if (valid && (S.sV || (ok && (code != 1) && (code != 3))))
+      store_cols<T, CPLX, E>(Vr, Vi, n, nEV, dp, dq, t, LP, vx, vy, vxi, vyi);
+    // (e,f) of the pair as pair_apply leaves them: exchanged for -1 / -3
+    if (valid && !err && (t == 0)) {
+      const bool xe = (code == -1) || (code == -3);
+      eS[p] = xe ? Pd.ey : Pd.ex;
+      fS[p] = xe ? Pd.fy : Pd.fx;
+      eS[q] = xe ? Pd.ex : Pd.ey;
+      fS[q] = xe ? Pd.fx : Pd.fy;
+    }
+    const unsigned long long eb = __ballot(valid && (err != 0));
+    if (eb)
+      werr = __builtin_amdgcn_readlane(err, (int)__builtin_ctzll(eb));
+    if (valid && !err)
+      wG = xmax(wG, tG);
+    wcnt += (unsigned long long)__popcll(__ballot(valid && !err && (t == 0) && D.trans));
+    if constexpr (PF) {
+      p = pn;
+      q = qn;
+      valid = validn;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        xr[j] = nxr[j]; yr[j] = nyr[j]; xi[j] = nxi[j]; yi[j] = nyi[j];
+        vx[j] = nvx[j]; vy[j] = nvy[j]; vxi[j] = nvxi[j]; vyi[j] = nvyi[j];
+      }
+    }
+  }
+}
+
+// Short columns (the launch of the short class, m <= 128 real, m <= 64
+// complex): 16 lanes per pair, four pairs per wave, 4 chunks up to 64 rows
+// and 8 beyond.  (Wider groups lose to the two-pairs-per-wave form of
+// wave_step, whose reductions scatter several sums over the wave.)
+template <typename T, bool CPLX, int E, int W>
+__device__ __forceinline__ void group_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
+                                               const unsigned st, const int wave, const int lane, const Step<T> &S,
+                                               T &wG, unsigned long long &wcnt, int &werr)
+{
+  if ((E < 2) || CPLX || (m <= 64)) {
+    group_step<T, CPLX, 4, W, GroupShape<T, CPLX, 4>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, 4, st, wave, lane, S, wG, wcnt,
+                                                          werr);
+    return;
+  }
+  if constexpr ((E >= 2) && !CPLX)
+    group_step<T, CPLX, 8, W, GroupShape<T, CPLX, 8>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, 4, st, wave, lane, S, wG, wcnt,
+                                                          werr);
+}
+
+// Longer columns: a pair per 64 lanes, two pairs per wave, with the register
+// footprint (64-row chunks per lane) of this matrix's row count.
+template <typename T, bool CPLX, int E, int W>
+__device__ __forceinline__ void wave_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
+                                              const unsigned st, const int wave, const int lane, const Step<T> &S,
+                                              T &wG, unsigned long long &wcnt, int &werr)
+{
+  const int nE = (m + 63) >> 6;
+  if ((E < 4) || (nE <= 2)) {
+    constexpr int E2 = (E >= 2) ? 2 : E;
+    wave_step<T, CPLX, E2, W, StepShape<T, CPLX, E2>::GP, StepShape<T, CPLX, E2>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
+                                                                                      wave, lane, S, wG, wcnt, werr);
+    return;
+  }
+  if constexpr (E >= 4) {
+    if ((E < 8) || (nE <= 4)) {
+      wave_step<T, CPLX, 4, W, StepShape<T, CPLX, 4>::GP, StepShape<T, CPLX, 4>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
+                                                                                     wave, lane, S, wG, wcnt, werr);
+      return;
+    }
+  }
+  if constexpr (E >= 8) {
+    if ((E < 16) || (nE <= 8)) {
+      wave_step<T, CPLX, 8, W, StepShape<T, CPLX, 8>::GP, StepShape<T, CPLX, 8>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
+                                                                                     wave, lane, S, wG, wcnt, werr);
+      return;
+    }
+  }
+  if constexpr (E >= 16)
+    wave_step<T, CPLX, E, W, StepShape<T, CPLX, E>::GP, StepShape<T, CPLX, E>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
+                                                                                   wave, lane, S, wG, wcnt, werr);
+}
+
 // Minimum waves per SIMD the register allocation must allow: two, which keeps
 // the double-buffered column pairs in registers, except for the widest
 // complex pairs, which would spill at 256 registers and run one wave per
@@ -400,10 +853,9 @@ template <typename T, bool CPLX, int E> struct BsvdMinWaves {
   static constexpr int value = (CPLX ? (E * (int)sizeof(T) >= 32) : (E * (int)sizeof(T) >= 128)) ? 1 : 2;
 };
 
-template <typename T, bool CPLX, int E, int W>
+template <typename T, bool CPLX, int E, int W, bool GRP>
 __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_bsvd(const BatchArgs<T> a)
 {
-  constexpr int B = 64;
   __shared__ int s_b;
   __shared__ T s_eS[128], s_fS[128];           // the matrix's (e,f) column norms (n <= 128)
   __shared__ double s_max[2][W][2];            // per-wave (tG, tV) maxima, by step parity
@@ -429,7 +881,6 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     T *const eS = a.eS + a.offS[b];
     T *const fS = a.fS + a.offS[b];
     const unsigned stp = (unsigned)n; // steps per sweep of the modulus ordering
-    const int nE = (m + 63) >> 6;
 
     // ---- setup (dvjsvd.c:78-184): max |G|, V := I, eS/fS, initial scale
     T mxv = (T)0;
@@ -496,19 +947,10 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           T wG = (T)0, wV = (T)0;
           unsigned long long wcnt = 0;
           int werr = 0;
-          // register footprint by this matrix's row count (chunks of 64 rows)
-          if ((E >= 2) && (nE <= 2))
-            wave_step<T, CPLX, (E >= 2 ? 2 : E), W, StepShape<T, CPLX, (E >= 2 ? 2 : E)>::GP, StepShape<T, CPLX, (E >= 2 ? 2 : E)>::PF>(
-                Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
-          else if ((E >= 4) && (nE <= 4))
-            wave_step<T, CPLX, (E >= 4 ? 4 : E), W, StepShape<T, CPLX, (E >= 4 ? 4 : E)>::GP, StepShape<T, CPLX, (E >= 4 ? 4 : E)>::PF>(
-                Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
-          else if ((E >= 8) && (nE <= 8))
-            wave_step<T, CPLX, (E >= 8 ? 8 : E), W, StepShape<T, CPLX, (E >= 8 ? 8 : E)>::GP, StepShape<T, CPLX, (E >= 8 ? 8 : E)>::PF>(
-                Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, nE, st, wave, lane, S, wG, wcnt, werr);
+          if constexpr (GRP)
+            group_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr);
           else
-            wave_step<T, CPLX, E, W, StepShape<T, CPLX, E>::GP, StepShape<T, CPLX, E>::PF>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, nE, st, wave, lane, S,
-                                                                   wG, wcnt, werr);
+            wave_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr);
           wG = red_max<T, 64>(wG, nullptr);
           if (lane == 0) {
             s_max[par][wave][0] = (double)wG;
@@ -588,21 +1030,21 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
   }
 }
 
-template <typename T, bool CPLX, int E, int W>
+template <typename T, bool CPLX, int E, int W, bool GRP>
 static int launch_bsvd(const BatchArgs<T> &a, int grid, hipStream_t s)
 {
-  hipLaunchKernelGGL((k_bsvd<T, CPLX, E, W>), dim3(grid), dim3(64 * W), 0, s, a);
+  hipLaunchKernelGGL((k_bsvd<T, CPLX, E, W, GRP>), dim3(grid), dim3(64 * W), 0, s, a);
   return check_launch();
 }
 
-template <typename T, bool CPLX, int E, int W>
+template <typename T, bool CPLX, int E, int W, bool GRP>
 static int occupancy(int *blocks)
 {
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_bsvd<T, CPLX, E, W>, 64 * W, 0);
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_bsvd<T, CPLX, E, W, GRP>, 64 * W, 0);
   return (e == hipSuccess) ? 0 : set_error(VJ_EHIP - (int)e, "occupancy query failed");
 }
 
-template <typename T, bool CPLX, int W>
+template <typename T, bool CPLX, int W, bool GRP>
 static int dispatch_E(const BatchArgs<T> &a, int max_m, int grid_cap, hipStream_t s)
 {
   int cus = 0, blocks = 1, dev = 0;
@@ -610,39 +1052,47 @@ static int dispatch_E(const BatchArgs<T> &a, int max_m, int grid_cap, hipStream_
   VJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
 #define VJ_BS_CASE(EE)                                                         \
   {                                                                            \
-    if (int rc = occupancy<T, CPLX, EE, W>(&blocks))                           \
+    if (int rc = occupancy<T, CPLX, EE, W, GRP>(&blocks))                      \
       return rc;                                                               \
     long long g = (long long)cus * (blocks > 0 ? blocks : 1);                  \
     if (grid_cap > 0 && g > grid_cap) g = grid_cap;                            \
     if (g > a.nb) g = a.nb;                                                    \
-    return launch_bsvd<T, CPLX, EE, W>(a, (int)g, s);                          \
+    return launch_bsvd<T, CPLX, EE, W, GRP>(a, (int)g, s);                     \
   }
   if (max_m <= 64) VJ_BS_CASE(1)
   if (max_m <= 128) VJ_BS_CASE(2)
-  if (max_m <= 256) VJ_BS_CASE(4)
-  if (max_m <= 512) VJ_BS_CASE(8)
-  if (max_m <= 1024) VJ_BS_CASE(16)
+  if constexpr (!GRP) {
+    if (max_m <= 256) VJ_BS_CASE(4)
+    if (max_m <= 512) VJ_BS_CASE(8)
+    if (max_m <= 1024) VJ_BS_CASE(16)
+  }
 #undef VJ_BS_CASE
-  return set_error(-1, "batched SVD: m = %d exceeds 1024", max_m);
+  return set_error(-1, "batched SVD: m = %d exceeds %d", max_m, GRP ? 128 : 1024);
+}
+
+template <typename T, bool CPLX, int W>
+static int dispatch_G(const BatchArgs<T> &a, int max_m, int grid_cap, bool grp, hipStream_t s)
+{
+  return grp ? dispatch_E<T, CPLX, W, true>(a, max_m, grid_cap, s) : dispatch_E<T, CPLX, W, false>(a, max_m, grid_cap, s);
 }
 
 template <typename T, bool CPLX>
-int batched_svd(const BatchArgs<T> &a, int waves, int max_m, int grid_cap, hipStream_t s)
+int batched_svd(const BatchArgs<T> &a, int waves, int max_m, int grid_cap, bool grp, hipStream_t s)
 {
   if (a.nb <= 0)
     return 0;
   switch (waves) {
-  case 1: return dispatch_E<T, CPLX, 1>(a, max_m, grid_cap, s);
-  case 2: return dispatch_E<T, CPLX, 2>(a, max_m, grid_cap, s);
-  case 4: return dispatch_E<T, CPLX, 4>(a, max_m, grid_cap, s);
-  case 8: return dispatch_E<T, CPLX, 8>(a, max_m, grid_cap, s);
+  case 1: return dispatch_G<T, CPLX, 1>(a, max_m, grid_cap, grp, s);
+  case 2: return dispatch_G<T, CPLX, 2>(a, max_m, grid_cap, grp, s);
+  case 4: return dispatch_G<T, CPLX, 4>(a, max_m, grid_cap, grp, s);
+  case 8: return dispatch_G<T, CPLX, 8>(a, max_m, grid_cap, grp, s);
   default: return set_error(-1, "batched SVD: waves per workgroup must be 1, 2, 4 or 8 (got %d)", waves);
   }
 }
 
-template int batched_svd<double, false>(const BatchArgs<double> &, int, int, int, hipStream_t);
-template int batched_svd<double, true>(const BatchArgs<double> &, int, int, int, hipStream_t);
-template int batched_svd<float, false>(const BatchArgs<float> &, int, int, int, hipStream_t);
-template int batched_svd<float, true>(const BatchArgs<float> &, int, int, int, hipStream_t);
+template int batched_svd<double, false>(const BatchArgs<double> &, int, int, int, bool, hipStream_t);
+template int batched_svd<double, true>(const BatchArgs<double> &, int, int, int, bool, hipStream_t);
+template int batched_svd<float, false>(const BatchArgs<float> &, int, int, int, bool, hipStream_t);
+template int batched_svd<float, true>(const BatchArgs<float> &, int, int, int, bool, hipStream_t);
 
 } // namespace vj

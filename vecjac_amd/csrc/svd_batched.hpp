@@ -18,6 +18,12 @@ template <typename T> struct BatchArgs {
   unsigned swp;                   // sweep limit
 };
 
-template <typename T, bool CPLX> int batched_svd(const BatchArgs<T> &a, int waves, int max_m, int grid_cap, hipStream_t s);
+// One launch over a.nb matrices of a.order: grp selects the short-column
+// form (lane groups of 16, m <= 128 real / 64 complex), else the wave form.
+template <typename T, bool CPLX>
+int batched_svd(const BatchArgs<T> &a, int waves, int max_m, int grid_cap, bool grp, hipStream_t s);
+
+// Rows up to which a matrix of a batch takes the short-column form.
+inline int short_rows(bool cplx) { return cplx ? 64 : 128; }
 
 } // namespace vj
