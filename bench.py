@@ -218,7 +218,7 @@ def bench_bsvd(args, kind: str = "d"):
         "dtype": "f64" if kind == "d" else "f32",
         "config": {"workload": f"batched {'FP64' if kind == 'd' else 'FP32'} one-sided Jacobi SVD of {nb} matrices, "
                                "n ~ U{8..128} (odd n padded with a zero column), m ~ U{n..4n}, N(0,1) entries, "
-                               "one persistent launch",
+                               "two persistent launches (m > 128 and m <= 128)",
                    "batch": nb, "waves_per_workgroup": r["waves"], "autotune_ms": r["tune"],
                    "sweeps_mean": float(np.mean(info)), "sweeps_max": int(np.max(info))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
