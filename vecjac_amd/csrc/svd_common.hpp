@@ -41,7 +41,6 @@ template <typename T> struct PlainState {
   int ef_init;                   // inputs accepted: V/eS/fS initialised
   T MG, MV;                      // running max-|.| bounds (dvjsvd.c:78, 88)
   unsigned long long MG_u, MV_u; // initial max-|.| (atomicMax on bit patterns)
-  unsigned long long nMG_u, nMV_u; // per-step maxima of updated elements
   int NRM_EXP;                   // exponent of max/(2m) (dvjsvd.c:120-123)
   int sT, sV;                    // accumulated power-of-two shifts
   int sG_init, sV_init;          // initial scalings
@@ -61,9 +60,19 @@ template <typename T> struct PlainState {
   } ev[32];
   unsigned err_sw, err_st;       // sweep and step of a failure
   // pair statistics (stats != 0): maxima of |cos(x, y)| over the pairs
-  // examined and of |sin| over the rotations applied, as bit patterns of
-  // non-negative doubles; folded into last_* at the end of every sweep
-  unsigned long long cos_u, sin_u, last_cos_u, last_sin_u;
+  // examined and of |sin| over the rotations applied, in the current sweep
+  // and in the last one completed
+  double cos_max, sin_max, last_cos, last_sin;
+};
+
+// What one workgroup of a step reports to the step's last workgroup, which
+// reduces all of them: one slot per pair, written with device-scope atomic
+// stores (no contended atomics on shared counters).
+struct PlainSlot {
+  double tG, tV;                 // max |updated G / V element|
+  double cosv, sinv;             // pair statistics (stats != 0)
+  unsigned trans;                // 1 if transformed
+  int err;                       // 0, or the reference's negative code
 };
 
 struct PlainRun {

@@ -761,6 +761,73 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
   return out;
 }
 
+// The V part of pair_apply for a V pair held in registers (row lane + B j,
+// j < nEV): the same swaps and rotations with the same fused multiply-adds;
+// returns the lane's max |updated element| (the caller reduces it when V
+// is tracked).  Nothing happens on a failed decision.
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ T v_apply(const PairDec<T> &D, T (&vx)[E], T (&vy)[E], T (&vxi)[E], T (&vyi)[E],
+                                     const int nEV)
+{
+  T tV = (T)0;
+  if (D.err)
+    return tV;
+  const int code = D.code;
+  const T c = D.c, at = D.at, sat = D.sat;
+  bool swap = (code == -1) || (code == -3);
+  if ((code == 2) || (code == -2)) {
+    const bool perm = (code < 0);
+    if ((at == (T)0) && (!CPLX || (sat == (T)0)))
+      swap = (c == (T)1) && perm;
+    else {
+      const bool c1 = (c == (T)1);
+      const T cna = -at;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        if (j >= nEV)
+          break;
+        const T x_r = vx[j], y_r = vy[j];
+        if (!CPLX || (sat == (T)0)) {
+          T a = perm ? fma_(-x_r, at, y_r) : fma_(y_r, at, x_r);
+          T b = perm ? fma_(y_r, at, x_r) : fma_(-x_r, at, y_r);
+          if (!c1) { a = a * c; b = b * c; }
+          vx[j] = a;
+          vy[j] = b;
+          tV = max1p(tV, max1p(vabs(a), vabs(b)));
+          if constexpr (CPLX) {
+            const T x_i = vxi[j], y_i = vyi[j];
+            T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
+            T b2 = perm ? fma_(y_i, at, x_i) : fma_(-x_i, at, y_i);
+            if (!c1) { a2 = a2 * c; b2 = b2 * c; }
+            vxi[j] = a2;
+            vyi[j] = b2;
+            tV = max1p(tV, max1p(vabs(a2), vabs(b2)));
+          }
+        }
+        else if constexpr (CPLX) {
+          const T x_i = vxi[j], y_i = vyi[j];
+          T ar = fma_(y_r, at, fma_(-y_i, sat, x_r));
+          T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
+          T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
+          T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
+          if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+          tV = xmax(tV, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
+          if (perm) { vx[j] = br; vxi[j] = bi; vy[j] = ar; vyi[j] = ai; }
+          else { vx[j] = ar; vxi[j] = ai; vy[j] = br; vyi[j] = bi; }
+        }
+      }
+    }
+  }
+  if (swap) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      T t = vx[j]; vx[j] = vy[j]; vy[j] = t;
+      if constexpr (CPLX) { t = vxi[j]; vxi[j] = vyi[j]; vyi[j] = t; }
+    }
+  }
+  return tV;
+}
+
 // The three phases back to back, for a group that owns one pair.
 template <typename T, bool CPLX, int B, int E>
 __device__ __forceinline__ PairOut<T, CPLX>

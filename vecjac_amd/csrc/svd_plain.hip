@@ -10,8 +10,6 @@
 // host round trip; once the flag is set, later launches return at once.
 #include "svd_common.hpp"
 
-#include <string.h>
-
 namespace vj {
 
 // ------------------------------------------------------------ setup kernels
@@ -50,7 +48,6 @@ __global__ void k_setup(const int m, PlainState<T> *st, const int vpreset)
   st->sG_init = st->sV_init = 0;
   st->sweeps = 0;
   st->swt = 0;
-  st->nMG_u = st->nMV_u = 0ull;
   st->blocks_done = 0u;
   st->zero = 0;
   st->status = 0;
@@ -185,7 +182,8 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
                                              const long long ldV, T *__restrict__ eS, T *__restrict__ fS,
                                              const unsigned *__restrict__ row, const unsigned st_idx,
                                              const unsigned stp, const unsigned swp, const T tol, const T gst,
-                                             const int vtrack, const int stats, PlainState<T> *__restrict__ st)
+                                             const int vtrack, const int stats, PlainState<T> *__restrict__ st,
+                                             PlainSlot *__restrict__ slots)
 {
   // n is the length of the V columns (the table row has gridDim.x pairs)
   __shared__ double lds[8 * (B / 64) + 2];
@@ -206,7 +204,11 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   T *const Gpi = CPLX ? Gi + (long long)p * ldG : nullptr, *const Gqi = CPLX ? Gi + (long long)q * ldG : nullptr;
   T *const Vp = Vr + (long long)p * ldV, *const Vq = Vr + (long long)q * ldV;
   T *const Vpi = CPLX ? Vi + (long long)p * ldV : nullptr, *const Vqi = CPLX ? Vi + (long long)q * ldV : nullptr;
-  if (S.sV) {
+  // the V pair in registers along with G (loaded at the start, so that its
+  // latency overlaps the reductions) when it fits E chunks of B rows (E <= 8,
+  // complex E <= 4)
+  const bool vreg = !STREAM && (E * (CPLX ? 2 : 1) <= 8) && (n <= E * B);
+  if (S.sV && !vreg) {
     for (int i = lane; i < n; i += B) {
       Vp[i] = ldexp_(Vp[i], S.sV);
       Vq[i] = ldexp_(Vq[i], S.sV);
@@ -219,7 +221,30 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   T eSp, fSp, eSq, fSq;
   PairOut<T, CPLX> o;
   PairDec<T> D;
-  if constexpr (!STREAM) {
+  if (vreg) {
+    T xr[E], yr[E], xi[E], yi[E], vx[E], vy[E], vxi[E], vyi[E];
+    const int nEV = (n + B - 1) / B;
+    load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, S.sG, lane, xr, yr, xi, yi);
+    load_rows<T, CPLX, B, E>(Vp, Vq, Vpi, Vqi, n, 0, S.sV, lane, vx, vy, vxi, vyi);
+    Step<T> Sg = S;
+    Sg.vtrack = 0;
+    o = pair_step<T, CPLX, B, E>(m, 0, xr, yr, xi, yi, nullptr, nullptr, nullptr, nullptr, eSp, fSp, eSq, fSq, Sg, lds,
+                                 lane, D);
+    if (!o.err && ((o.code != 1) || S.sG))
+      store_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, lane, xr, yr, xi, yi);
+    if (!D.err) {
+      T tV = v_apply<T, CPLX, E>(D, vx, vy, vxi, vyi, nEV);
+      if (vtrack && ((D.code == 2) || (D.code == -2)) && !((D.at == (T)0) && (!CPLX || (D.sat == (T)0)))) {
+        tV = red_max<T, B>(tV, lds);
+        o.tV = tV;
+        if (!o.err && !isfinite_(tV))
+          o.err = CPLX ? -638 : -588;
+      }
+    }
+    if (S.sV || (!D.err && (D.code != 1) && (D.code != 3)))
+      store_rows<T, CPLX, B, E>(Vp, Vq, Vpi, Vqi, n, 0, lane, vx, vy, vxi, vyi);
+  }
+  else if (!STREAM) {
     T xr[E], yr[E], xi[E], yi[E];
     load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, S.sG, lane, xr, yr, xi, yi);
     o = pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, lds, lane, D);
@@ -268,45 +293,95 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     }
   }
   if (lane == 0) {
-    if (o.err)
-      atomicCAS(&st->status, 0, o.err);
-    else {
+    // this pair's report, then the ticket; the wait makes the slot words
+    // complete before the ticket is taken
+    PlainSlot *const sl = slots + blockIdx.x;
+    __hip_atomic_store(&sl->err, o.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!o.err) {
       eS[p] = eSp; fS[p] = fSp;
       eS[q] = eSq; fS[q] = fSq;
-      atomicMax(&st->nMG_u, (unsigned long long)__double_as_longlong((double)o.tG));
-      atomicMax(&st->nMV_u, (unsigned long long)__double_as_longlong((double)o.tV));
-      if (o.trans)
-        atomicAdd(&st->swt, 1ull);
+      __hip_atomic_store(&sl->tG, (double)o.tG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&sl->tV, (double)o.tV, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&sl->trans, (unsigned)o.trans, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (stats) {
         // |sin| of the rotation: c |tan|, the complex tangent being at + i sat
         T ta = vabs(D.at);
         if constexpr (CPLX)
           ta = sqrt_(fma_(D.sat, D.sat, ta * ta));
         const T sn = D.trans ? D.c * ta : (T)0;
-        atomicMax(&st->cos_u, (unsigned long long)__double_as_longlong((double)D.cosv));
-        atomicMax(&st->sin_u, (unsigned long long)__double_as_longlong((double)sn));
+        __hip_atomic_store(&sl->cosv, (double)D.cosv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&sl->sinv, (double)sn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    __threadfence();
-    last = (atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1u);
+    __builtin_amdgcn_s_waitcnt(0);
+    last = (__hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u);
   }
   __syncthreads();
-  if (!last || lane)
+  if (!last)
     return;
-  // ---- last workgroup of the step: fold the maxima, decide the next
-  // step's rescaling (dvjsvd.c:198-233) and the end of the sweep (:599-601)
-  __threadfence();
+  // ---- last workgroup of the step: reduce the slots (all lanes), fold the
+  // maxima, decide the next step's rescaling (dvjsvd.c:198-233) and the end
+  // of the sweep (:599-601)
+  double rG = 0.0, rV = 0.0, rc = 0.0, rs = 0.0;
+  unsigned long long rt = 0ull, rkey = ~0ull; // first failing pair: (index, code)
+  for (int i = lane; i < (int)gridDim.x; i += B) {
+    const PlainSlot *const sl = slots + i;
+    const int e = __hip_atomic_load(&sl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (e) {
+      const unsigned long long key = ((unsigned long long)(unsigned)i << 32) | (unsigned)e;
+      rkey = (key < rkey) ? key : rkey;
+      continue;
+    }
+    rG = fmax(rG, __hip_atomic_load(&sl->tG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    rV = fmax(rV, __hip_atomic_load(&sl->tV, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    rt += __hip_atomic_load(&sl->trans, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (stats) {
+      rc = fmax(rc, __hip_atomic_load(&sl->cosv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      rs = fmax(rs, __hip_atomic_load(&sl->sinv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+  }
+#pragma unroll
+  for (int o2 = 32; o2 >= 1; o2 >>= 1) {
+    rG = fmax(rG, __shfl_xor(rG, o2, 64));
+    rV = fmax(rV, __shfl_xor(rV, o2, 64));
+    rc = fmax(rc, __shfl_xor(rc, o2, 64));
+    rs = fmax(rs, __shfl_xor(rs, o2, 64));
+    rt += __shfl_xor(rt, o2, 64);
+    const unsigned long long k2 = __shfl_xor(rkey, o2, 64);
+    rkey = (k2 < rkey) ? k2 : rkey;
+  }
+  if constexpr (B > 64) {
+    double *const w = lds + 6 * (lane >> 6);
+    if ((lane & 63) == 0) {
+      w[0] = rG; w[1] = rV; w[2] = rc; w[3] = rs;
+      w[4] = __longlong_as_double((long long)rt);
+      w[5] = __longlong_as_double((long long)rkey);
+    }
+    __syncthreads();
+    for (int w2 = 1; w2 < B / 64; ++w2) {
+      const double *const u = lds + 6 * w2;
+      rG = fmax(rG, u[0]); rV = fmax(rV, u[1]); rc = fmax(rc, u[2]); rs = fmax(rs, u[3]);
+      rt += (unsigned long long)__double_as_longlong(u[4]);
+      const unsigned long long k2 = (unsigned long long)__double_as_longlong(u[5]);
+      rkey = (k2 < rkey) ? k2 : rkey;
+    }
+  }
+  if (lane)
+    return;
   st->blocks_done = 0u;
-  if (st->status) {
+  if (rkey != ~0ull) {
+    st->status = (int)(unsigned)(rkey & 0xffffffffull);
     st->err_sw = st->sweeps;
     st->err_st = st_idx;
     st->done = 1;
     return;
   }
-  const T nMG = (T)__longlong_as_double((long long)st->nMG_u);
-  const T nMV = (T)__longlong_as_double((long long)st->nMV_u);
-  st->nMG_u = 0ull;
-  st->nMV_u = 0ull;
+  const T nMG = (T)rG, nMV = (T)rV;
+  st->swt += rt;
+  if (stats) {
+    st->cos_max = fmax(st->cos_max, rc);
+    st->sin_max = fmax(st->sin_max, rs);
+  }
   // st->MG already carries this step's shift (applied when it was decided)
   T MG = st->MG;
   MG = (nMG > MG) ? nMG : MG;
@@ -346,9 +421,9 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   st->sV_next = sV;
   if (st_idx == stp - 1u) {
     if (stats) {
-      st->last_cos_u = st->cos_u;
-      st->last_sin_u = st->sin_u;
-      st->cos_u = st->sin_u = 0ull;
+      st->last_cos = st->cos_max;
+      st->last_sin = st->sin_max;
+      st->cos_max = st->sin_max = 0.0;
     }
     const unsigned long long swt = st->swt;
     if (st->sweeps < 64u)
@@ -397,16 +472,18 @@ __global__ __launch_bounds__(256) void k_epilogue(const int m, const int n, T *_
 
 // ------------------------------------------------------------------ driver
 template <typename T, bool CPLX, int B, int E, bool STREAM = false>
-static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, const unsigned *row, unsigned s, const T tol, const T gst, hipStream_t strm)
+static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, PlainSlot *slots, const unsigned *row, unsigned s,
+                        const T tol, const T gst, hipStream_t strm)
 {
   hipLaunchKernelGGL((k_step<T, CPLX, B, E, STREAM>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m,
                      (int)(a.nv ? a.nv : a.n), a.Gr, a.Gi, a.ldG, a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp,
-                     tol, gst, a.vpreset, a.stats, st);
+                     tol, gst, a.vpreset, a.stats, st, slots);
   return 0;
 }
 
 template <typename T, bool CPLX>
-using StepFn = int (*)(const PlainArgs<T> &, PlainState<T> *, const unsigned *, unsigned, T, T, hipStream_t);
+using StepFn = int (*)(const PlainArgs<T> &, PlainState<T> *, PlainSlot *, const unsigned *, unsigned, T, T,
+                       hipStream_t);
 
 template <typename T, bool CPLX>
 static StepFn<T, CPLX> pick_step(long long m, int &B_out)
@@ -431,12 +508,6 @@ static StepFn<T, CPLX> pick_step(long long m, int &B_out)
 }
 
 template <typename T> static T ldexp_host(T x, int e) { return std::ldexp(x, e); }
-static double bits_to_double(unsigned long long u)
-{
-  double d;
-  memcpy(&d, &u, sizeof(d));
-  return d;
-}
 
 template <typename T, bool CPLX>
 int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
@@ -447,8 +518,14 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   StepFn<T, CPLX> step = pick_step<T, CPLX>(a.m, B);
   if (!step)
     return set_error(-1, "m = %lld exceeds the largest supported column length", a.m);
+  // control block and per-pair slots, released in stream order on return
+  Staging sg(s);
   PlainState<T> *st = nullptr;
-  VJ_HIP(hipMallocAsync((void **)&st, sizeof(PlainState<T>), s));
+  PlainSlot *slots = nullptr;
+  if (int rc = sg.alloc(&st, 1))
+    return rc;
+  if (int rc = sg.alloc(&slots, (size_t)(a.n / 2)))
+    return rc;
   VJ_HIP(hipMemsetAsync(st, 0, sizeof(PlainState<T>), s));
   const int mx_blocks = 1024;
   hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gr, a.ldG, &st->MG_u);
@@ -482,7 +559,7 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
     rc = set_error(VJ_EHIP, "event creation failed");
   for (unsigned sw = 0; !rc && sw < a.swp; ++sw) {
     for (unsigned k = 0; k < a.stp; ++k)
-      step(a, st, a.js + (size_t)k * a.n, k, tol, gst, s);
+      step(a, st, slots, a.js + (size_t)k * a.n, k, tol, gst, s);
     if ((rc = check_launch()))
       break;
     ++launched;
@@ -521,7 +598,6 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   (void)hipHostFree(hf);
-  (void)hipFreeAsync(st, s);
   if (rc)
     return rc;
   if (run) {
@@ -551,8 +627,8 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
     run->err_sw = h.err_sw;
     run->err_st = h.err_st;
     run->steps_ms = steps_ms;
-    run->last_cos = bits_to_double(h.last_cos_u);
-    run->last_sin = bits_to_double(h.last_sin_u);
+    run->last_cos = h.last_cos;
+    run->last_sin = h.last_sin;
     run->sG_pending = h.sG_next;
     run->sV_pending = h.sV_next;
   }
