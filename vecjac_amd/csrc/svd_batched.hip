@@ -245,8 +245,10 @@ template <typename T> __device__ __forceinline__ PairScal<T> lane_scal(const Pai
 }
 template <typename T> __device__ __forceinline__ PairDec<T> lane_dec(const PairDec<T> &D, const int r)
 {
-  return PairDec<T>{__builtin_amdgcn_readlane(D.code, r), __builtin_amdgcn_readlane(D.trans, r),
-                    __builtin_amdgcn_readlane(D.err, r), readlane_(D.c, r), readlane_(D.at, r), readlane_(D.sat, r)};
+  // code (-3..3), trans (0/1) and err (0 or minus a line number) in one word
+  const int pk = (D.code + 8) | (D.trans << 4) | ((-D.err) << 5);
+  const int u = __builtin_amdgcn_readlane(pk, r);
+  return PairDec<T>{(u & 15) - 8, (u >> 4) & 1, -(u >> 5), readlane_(D.c, r), readlane_(D.at, r), readlane_(D.sat, r)};
 }
 
 // One step of the strategy for the pairs of one wave (pairs wave, wave + W,
@@ -350,8 +352,12 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
     for (int r = 0; r < GP; ++r) {
       if (g * GP + r >= cnt)
         break;
-      const PairScal<T> P = lane_scal(Pl, r);
       const PairDec<T> D = lane_dec(Dl, r);
+      // the scalars are needed on every lane only by a Gram-Schmidt update;
+      // otherwise they only give the (e,f) outputs, written by lane r
+      // (which holds pair r's own values)
+      const bool gsr = (D.code == 3) || (D.code == -3);
+      const PairScal<T> P = gsr ? lane_scal(Pl, r) : Pl;
       T eSp, fSp, eSq, fSq;
       const PairOut<T, CPLX> o = pair_apply<T, CPLX, B, E, false>(0, xr[r], yr[r], xi[r], yi[r], nullptr, nullptr, nullptr,
                                                                   nullptr, P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane, nE);
@@ -372,7 +378,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
       if (err)
         werr = err;
       else {
-        if (lane == 0) {
+        if (lane == r) {
           eS[p[r]] = eSp; fS[p[r]] = fSp;
           eS[q[r]] = eSq; fS[q[r]] = fSq;
         }
