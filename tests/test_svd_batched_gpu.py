@@ -1,8 +1,9 @@
-"""GPU parity of the batched SVD (one persistent launch for many uneven
-matrices).  Every matrix is checked against the oracle on the same input
-with the same step table (sigma within 8 n eps, long-double orthogonality and
-reconstruction) and against the plain GPU driver, which runs the same
-wave-level pair step for m <= 512 and must agree bit for bit."""
+"""GPU parity of the batched SVD (many uneven matrices in two persistent
+launches: columns longer than 128 rows (64 complex) in the wave form, the
+rest in the lane-group form).  Every matrix is checked against the oracle on
+the same input with the same step table (sigma within 8 n eps, long-double
+orthogonality and reconstruction) and against the plain GPU driver, which
+performs the same operations per pair and must agree bit for bit."""
 import numpy as np
 import pytest
 
@@ -48,7 +49,7 @@ def _run(kind, m, npad, G, tot, waves=0, Gi=None):
     return out
 
 
-@pytest.mark.parametrize("waves", [0, 1, 4])
+@pytest.mark.parametrize("waves", [0, 1, 4, 8])
 def test_batched_vs_oracle_f64(waves):
     nb = 48
     m, n, npad, offG, offV, offS, tot, G = _batch(nb, 100 + waves)
@@ -239,3 +240,41 @@ def test_batched_autotune_keeps_results():
     assert np.array_equal(Gd.cpu().numpy(), base["G"], equal_nan=True)
     assert np.array_equal(eS.cpu().numpy(), base["eS"]) and np.array_equal(fS.cpu().numpy(), base["fS"])
     plan.close()
+
+
+@pytest.mark.parametrize("kind", ["d", "z"])
+def test_short_form_vs_plain_bitwise(kind):
+    """Members on both sides of the lane-group bounds (4 chunks of 16 lanes up
+    to 64 rows, 8 chunks up to 128; complex up to 64 rows) and just past them
+    in the wave form, against the plain driver bit for bit."""
+    import torch
+    m = np.array([8, 16, 40, 63, 64, 65, 96, 127, 128, 129, 200], dtype=np.int64)
+    n = np.array([8, 10, 40, 30, 64, 34, 48, 64, 128, 66, 100], dtype=np.int64)
+    _, npad, offG, offV, offS, tot = BT.pack_shapes(m, n)
+    G = BT.fill_gaussian(m, npad, n, offG, 431)
+    Gi = BT.fill_gaussian(m, npad, n, offG, 432) if kind == "z" else None
+    r = _run(kind, m, npad, G, tot, Gi=Gi)
+    assert r["rc"] == 0 and np.all(r["info"] > 0)
+    for b in range(len(m)):
+        mm, nn = int(m[b]), int(npad[b])
+        js, stp = S.pjs(4, nn)
+        Gd = torch.from_numpy(G[offG[b]: offG[b] + mm * nn].reshape(nn, mm).copy()).cuda()
+        Vd = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
+        Gid = Vid = None
+        if kind == "z":
+            Gid = torch.from_numpy(Gi[offG[b]: offG[b] + mm * nn].reshape(nn, mm).copy()).cuda()
+            Vid = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
+        eS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        fS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        jsd = torch.from_numpy(js.reshape(-1).view(np.int32)).cuda()
+        print(f"plain {kind} m={mm} n={nn}", flush=True)
+        rc, _ = S.device_vjsvd(kind, Gd, Vd, eS, fS, jsd, stp, Gi=Gid, Vi=Vid)
+        assert rc == int(r["info"][b]), (b, rc, r["info"][b])
+        sl = slice(offG[b], offG[b] + mm * nn)
+        assert np.array_equal(Gd.cpu().numpy().reshape(-1), r["G"][sl], equal_nan=True), b
+        assert np.array_equal(Vd.cpu().numpy().reshape(-1), r["V"][offV[b]: offV[b] + nn * nn]), b
+        if kind == "z":
+            assert np.array_equal(Gid.cpu().numpy().reshape(-1), r["Gi"][sl], equal_nan=True), b
+            assert np.array_equal(Vid.cpu().numpy().reshape(-1), r["Vi"][offV[b]: offV[b] + nn * nn]), b
+        assert np.array_equal(eS.cpu().numpy(), r["eS"][offS[b]: offS[b] + nn]), b
+        assert np.array_equal(fS.cpu().numpy(), r["fS"][offS[b]: offS[b] + nn]), b
