@@ -31,29 +31,50 @@ __global__ __launch_bounds__(256) void k_svd_bwd(const int nb, const int m, cons
   T *const sds = ss + n;              // n
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     const long long oU = (long long)b * m * n, oV = (long long)b * n * n, oS = (long long)b * n;
+    // dV is staged in sJ until K has been formed
     for (int t = threadIdx.x; t < m * n; t += blockDim.x) {
       sU[t] = U[oU + t];
       sdU[t] = dU[oU + t];
     }
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x)
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
       sV[t] = V[oV + t];
+      sJ[t] = dV[oV + t];
+    }
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
       ss[t] = s[oS + t];
       sds[t] = ds[oS + t];
     }
     __syncthreads();
-    // J = U^T dU and K = V^T dV (K kept in sN for now; dV read from memory)
+    // K = V^T dV into sN.  Lane (i, j) runs its sum from row i on (r' =
+    // (r + i) mod len), so that the lanes of a wave, which differ in i, read
+    // different LDS banks instead of one column stride apart.
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
       const int i = t % n, j = t / n; // element (i, j), column-major
-      T acc = (T)0;
-      for (int r = 0; r < m; ++r)
-        acc = fma_(sU[i * m + r], sdU[j * m + r], acc);
-      sJ[t] = acc;
       T acck = (T)0;
-      for (int r = 0; r < n; ++r)
-        acck = fma_(sV[i * n + r], dV[oV + j * n + r], acck);
+      int r = i % n;
+      for (int k = 0; k < n; ++k) {
+        acck = fma_(sV[i * n + r], sJ[j * n + r], acck);
+        r = (r + 1 == n) ? 0 : (r + 1);
+      }
       sN[t] = acck;
     }
+    __syncthreads();
+    // J = U^T dU into sJ (dV is no longer needed)
+    T jv[16];
+    int cj = 0;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+      const int i = t % n, j = t / n;
+      T acc = (T)0;
+      int r = i % m;
+      for (int k = 0; k < m; ++k) {
+        acc = fma_(sU[i * m + r], sdU[j * m + r], acc);
+        r = (r + 1 == m) ? 0 : (r + 1);
+      }
+      jv[cj++] = acc;
+    }
+    cj = 0;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x)
+      sJ[t] = jv[cj++];
     __syncthreads();
     // N_ij
     T nv[16];
