@@ -255,22 +255,69 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     typedef typename Acc<T>::A A;
     const int nc = (m + B * E - 1) / (B * E); // chunks of E rows per lane
     T xr[E], yr[E], xi[E], yi[E];
-    T mx = (T)0, my = (T)0;
+    // The sums are taken in the same pass as the maxima, scaled by a
+    // predicted exponent: the norm exponent each column was left with (every
+    // step evaluates every column), moved by a pending rescaling.  Power-of-
+    // two scaling commutes with every rounding of the sums as long as no
+    // product or error term leaves the normal range under either exponent,
+    // which holds when every nonzero element scaled by either exponent lies
+    // in [2^-440, 2^100]; the sums are then rescaled to the max-exponent
+    // scaling exactly, and otherwise taken again in a second pass.
+    const T epx = eS[p], epy = eS[q];
+    const bool pred = isfinite_(epx) && isfinite_(epy);
+    const int kpx = pred ? (int)epx + S.sG : 0, kpy = pred ? (int)epy + S.sG : 0;
+    T mx = (T)0, my = (T)0, nx = inf_<T>(), ny = inf_<T>(); // max and min nonzero |element|
+    DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
     for (int c = 0; c < nc; ++c) {
       load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
       T cx, cy;
       pair_maxima<T, CPLX, E>(xr, yr, xi, yi, E, cx, cy);
       mx = __builtin_fmax(mx, cx);
       my = __builtin_fmax(my, cy);
+      if (pred) {
+        pair_sums<T, CPLX, E>(xr, yr, xi, yi, E, kpx, kpy, sx, sy, dr, di);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          nx = __builtin_fmin(nx, (xr[j] != (T)0) ? vabs(xr[j]) : inf_<T>());
+          ny = __builtin_fmin(ny, (yr[j] != (T)0) ? vabs(yr[j]) : inf_<T>());
+          if constexpr (CPLX) {
+            nx = __builtin_fmin(nx, (xi[j] != (T)0) ? vabs(xi[j]) : inf_<T>());
+            ny = __builtin_fmin(ny, (yi[j] != (T)0) ? vabs(yi[j]) : inf_<T>());
+          }
+        }
+      }
     }
     red_max2<T, B>(mx, my, lds);
     const int kx = max_exp(mx), ky = max_exp(my);
-    DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
-    for (int c = 0; c < nc; ++c) {
-      load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
-      pair_sums<T, CPLX, E>(xr, yr, xi, yi, E, kx, ky, sx, sy, dr, di);
+    bool one = false;
+    if (pred) {
+      T mnx = -nx, mny = -ny;
+      red_max2<T, B>(mnx, mny, lds);
+      auto fits = [](const T mn, const T mxv, const int k, const int kp) {
+        if (!(mxv > (T)0))
+          return true; // a zero column: every sum is zero under any scaling
+        const int en = frexp_exp(-mn) - 1;
+        const int hi = (k > kp) ? k : kp;
+        return (en - hi >= -440) && (k - kp <= 100);
+      };
+      one = fits(mnx, mx, kx, kpx) && fits(mny, my, ky, kpy);
     }
-    red_dd4<A, B>(sx, sy, dr, di, lds);
+    if (one) {
+      red_dd4<A, B>(sx, sy, dr, di, lds);
+      const int dx = kpx - kx, dy = kpy - ky;
+      sx = DD<A>{ldexp_(sx.s, 2 * dx), ldexp_(sx.c, 2 * dx)};
+      sy = DD<A>{ldexp_(sy.s, 2 * dy), ldexp_(sy.c, 2 * dy)};
+      dr = DD<A>{ldexp_(dr.s, dx + dy), ldexp_(dr.c, dx + dy)};
+      di = DD<A>{ldexp_(di.s, dx + dy), ldexp_(di.c, dx + dy)};
+    }
+    else {
+      sx = sy = dr = di = DD<A>{0, 0};
+      for (int c = 0; c < nc; ++c) {
+        load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
+        pair_sums<T, CPLX, E>(xr, yr, xi, yi, E, kx, ky, sx, sy, dr, di);
+      }
+      red_dd4<A, B>(sx, sy, dr, di, lds);
+    }
     const PairScal<T> P = pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
     D = pair_decide<T, CPLX>(P, S.tol, S.gst);
     T tG = (T)0;
