@@ -212,3 +212,24 @@ def test_plain_host_argument_codes():
     bad = js.copy()
     bad[0, 0] = bad[0, 1]
     assert call(js=bad) == -9         # not a permutation
+
+
+@pytest.mark.parametrize("kind", ["d", "z"])
+def test_streamed_wide_range_and_zero_column(kind):
+    """Streamed columns (m > 8192) whose elements span more than 2^440: the
+    one-pass sums under the predicted exponent fail the range check there and
+    the sums are taken again; a zero column has no predicted exponent."""
+    m, n = 9000, 8
+    A, Ai = _mat(kind, m, n, seed=77)
+    A[0, :100] *= 2.0 ** -600       # tiny elements in a column with normal ones
+    A[3, :] = 0.0                   # a zero column
+    if Ai is not None:
+        Ai[0, :100] *= 2.0 ** -600
+        Ai[3, :] = 0.0
+    g = _run_gpu(kind, A, Ai)
+    o = _run_oracle(kind, A, Ai)
+    assert g["rc"] >= 0 and abs(g["rc"] - o["rc"]) <= 1, (g["rc"], o["rc"])
+    sg, so = np.sort(S.sigma(g["eS"], g["fS"])), np.sort(S.sigma(o["eS"], o["fS"]))
+    nz = so > 0
+    assert np.all(sg[~nz] == 0) and np.sum(~nz) == 1
+    assert np.max(np.abs(sg[nz] - so[nz]) / so[nz]) <= 8 * n * EPS[kind]
