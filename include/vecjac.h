@@ -166,7 +166,8 @@ vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, floa
 
 /* ---- 3. batched SVD and SVD adjoint (no reference counterpart) --------- */
 
-/* Plan for a batch of nb independent SVDs solved in one persistent launch.
+/* Plan for a batch of nb independent SVDs solved in two persistent launches
+ * (matrices with m > 128, m > 64 complex, first; then the rest).
  * kind: 0 = FP64 real, 1 = FP64 complex (split re/im), 2 = FP32 real,
  * 3 = FP32 complex.  Host descriptors: m[b], n[b] (n even, 2 <= n <= 128,
  * n <= m <= 1024), element offsets offG[b] of G_b (ld m[b]), offV[b] of V_b (ld n[b])
@@ -175,7 +176,7 @@ vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, floa
  * failure with *status < 0. */
 void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG,
                        const vj_fint *offV, const vj_fint *offS, unsigned swp, int waves, vj_fint *status);
-/* One launch on device buffers: G (A in, U out; U columns of zero singular
+/* One run (the two launches) on device buffers: G (A in, U out; U columns of zero singular
  * values are NaN as in the reference), V (out, starts as I), eS/fS (out),
  * info[b] (sweeps or a negative code), transforms[b] (rotations and
  * Gram-Schmidt steps of matrix b; may be NULL).  Gi/Vi are NULL for real
