@@ -1,4 +1,4 @@
-"""Batched Jacobi SVD of many uneven matrices in one persistent launch.
+"""Batched Jacobi SVD of many uneven matrices (two persistent launches per run).
 
 Layout (device memory, one buffer per quantity): matrix b has its m_b x n_b
 G (A on entry, U on exit) at element offset offG[b], column-major with
