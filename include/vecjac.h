@@ -186,10 +186,11 @@ vj_fint vjh_bplan_run(void *plan, void *G, void *Gi, void *V, void *Vi, void *eS
 int vjh_bplan_waves(const void *plan);
 /* Autotuner of a plan: runs it with 1, 2, 4 and 8 waves per workgroup (counts
  * that would spill registers are skipped) on a copy of the input G (Gi) made
- * into the scratch buffer Gw (Gwi), each timed with HIP events on `stream`,
- * and keeps the fastest for later runs.  ms[4] (may be NULL) receives the
- * times, -1 for skipped counts.  V, eS, fS and info are overwritten.  Returns
- * the chosen count, or a negative code.  Synchronises `stream`. */
+ * into the scratch buffer Gw (Gwi), timing each of the two launches with HIP
+ * events on `stream`, and keeps the fastest count of each launch for later
+ * runs.  ms[4] (may be NULL) receives the run times, -1 for skipped counts.
+ * V, eS, fS and info are overwritten.  Returns the count chosen for the
+ * long-column launch, or a negative code.  Synchronises `stream`. */
 vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
                            void *fS, vj_fint *info, float *ms, void *stream);
 void vjh_bplan_destroy(void *plan);

@@ -23,7 +23,8 @@ struct BPlan {
   int max_m = 0;
   int nlong = 0;           // matrices with m > short_rows: first in the order, one launch
   int max_m_long = 0, max_m_short = 0;
-  int waves = 4;
+  int waves = 4;           // waves per workgroup of the long-column launch
+  int waves_short = 4;     // and of the short-column launch
   long long totG = 0;      // elements of the packed G (max over b of offG + m n)
   unsigned swp = 999u;
   int dev = 0;
@@ -56,9 +57,10 @@ int cap_waves(int kind, int max_m, int waves)
   return (waves > 4 && bytes > limit) ? 4 : waves;
 }
 
+// mid, if not null, is recorded between the two launches (autotuner timing)
 template <typename T, bool CPLX>
 int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vj_fint *info,
-             uint64_t *transforms, hipStream_t s)
+             uint64_t *transforms, hipStream_t s, hipEvent_t mid = nullptr)
 {
   BatchArgs<T> a{};
   const long long nb = P->nb;
@@ -86,10 +88,23 @@ int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, v
   a.queue = P->d_queue;
   if (int rc = batched_svd<T, CPLX>(a, P->waves, P->max_m_long, 0, false, s))
     return rc;
+  if (mid && hipEventRecord(mid, s) != hipSuccess)
+    return set_error(VJ_EHIP, "event record failed");
   a.nb = P->nb - P->nlong;
   a.order = P->d_order + P->nlong;
   a.queue = P->d_queue + 1;
-  return batched_svd<T, CPLX>(a, P->waves, P->max_m_short, 0, true, s);
+  return batched_svd<T, CPLX>(a, P->waves_short, P->max_m_short, 0, true, s);
+}
+
+int run_any(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vj_fint *info,
+            uint64_t *transforms, hipStream_t s, hipEvent_t mid)
+{
+  switch (P->kind) {
+  case 0: return run_plan<double, false>(P, Gr, nullptr, Vr, nullptr, eS, fS, info, transforms, s, mid);
+  case 1: return run_plan<double, true>(P, Gr, Gi, Vr, Vi, eS, fS, info, transforms, s, mid);
+  case 2: return run_plan<float, false>(P, Gr, nullptr, Vr, nullptr, eS, fS, info, transforms, s, mid);
+  default: return run_plan<float, true>(P, Gr, Gi, Vr, Vi, eS, fS, info, transforms, s, mid);
+  }
 }
 
 } // namespace
@@ -126,7 +141,8 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   P->max_m = max_m;
   P->totG = totG;
   P->swp = swp;
-  P->waves = cap_waves(kind, max_m, (waves > 0) ? waves : (nb ? pick_waves(hn) : 4));
+  const int w0 = (waves > 0) ? waves : (nb ? pick_waves(hn) : 4);
+  P->waves = cap_waves(kind, max_m, w0);
   (void)hipGetDevice(&P->dev);
   // processing order: the long-column class, then the short one, each by
   // decreasing m n^2 (longest first)
@@ -147,6 +163,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
     else
       P->max_m_short = std::max(P->max_m_short, (int)hm[b]);
   }
+  P->waves_short = cap_waves(kind, P->max_m_short, w0);
   std::vector<long long> meta;
   meta.reserve((size_t)(5 * nb));
   meta.insert(meta.end(), hm.begin(), hm.end());
@@ -177,13 +194,7 @@ vj_fint vjh_bplan_run(void *plan, void *Gr, void *Gi, void *Vr, void *Vi, void *
   const BPlan *P = (const BPlan *)plan;
   if (!P)
     return set_error(-1, "null plan");
-  hipStream_t s = (hipStream_t)stream;
-  switch (P->kind) {
-  case 0: return run_plan<double, false>(P, Gr, nullptr, Vr, nullptr, eS, fS, info, transforms, s);
-  case 1: return run_plan<double, true>(P, Gr, Gi, Vr, Vi, eS, fS, info, transforms, s);
-  case 2: return run_plan<float, false>(P, Gr, nullptr, Vr, nullptr, eS, fS, info, transforms, s);
-  default: return run_plan<float, true>(P, Gr, Gi, Vr, Vi, eS, fS, info, transforms, s);
-  }
+  return run_any(P, Gr, Gi, Vr, Vi, eS, fS, info, transforms, (hipStream_t)stream, nullptr);
 }
 
 int vjh_bplan_waves(const void *plan) { return plan ? ((const BPlan *)plan)->waves : 0; }
@@ -201,38 +212,57 @@ vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, 
   VJ_HIP(hipEventCreate(&e0));
   VJ_HIP(hipEventCreate(&e1));
   static const int cand[4] = {1, 2, 4, 8};
-  int best = P->waves;
-  float best_ms = -1.0f;
+  // each launch is timed on its own (events before, between and after the
+  // two launches) and keeps its own fastest count
+  hipEvent_t em = nullptr;
+  VJ_HIP(hipEventCreate(&em));
+  int best = P->waves, best_s = P->waves_short;
+  float best_ms = -1.0f, best_ms_s = -1.0f;
   vj_fint rc = 0;
   for (int k = 0; k < 4 && !rc; ++k) {
     const int w = cap_waves(P->kind, P->max_m, cand[k]);
-    float t = -1.0f;
-    if (w == cand[k]) { // a capped candidate is the same launch as a smaller one
-      P->waves = w;
-      if (bytes) {
-        if ((rc = (vj_fint)hipMemcpyAsync(Gw, G, bytes, hipMemcpyDeviceToDevice, s)))
-          rc = set_error(VJ_EHIP - (int)rc, "autotune: copy failed");
-        if (!rc && cplx && (rc = (vj_fint)hipMemcpyAsync(Gwi, Gi, bytes, hipMemcpyDeviceToDevice, s)))
-          rc = set_error(VJ_EHIP - (int)rc, "autotune: copy failed");
-      }
-      if (!rc && (rc = (vj_fint)hipEventRecord(e0, s)))
-        rc = set_error(VJ_EHIP - (int)rc, "autotune: event failed");
-      if (!rc)
-        rc = vjh_bplan_run(P, Gw, Gwi, V, Vi, eS, fS, info, nullptr, s);
-      if (!rc && (rc = (vj_fint)hipEventRecord(e1, s)))
-        rc = set_error(VJ_EHIP - (int)rc, "autotune: event failed");
-      if (!rc && (rc = (vj_fint)hipEventSynchronize(e1)))
-        rc = set_error(VJ_EHIP - (int)rc, "autotune: synchronisation failed");
-      if (!rc && (rc = (vj_fint)hipEventElapsedTime(&t, e0, e1)))
-        rc = set_error(VJ_EHIP - (int)rc, "autotune: timing failed");
-      if (!rc && (best_ms < 0.0f || t < best_ms)) {
-        best_ms = t;
+    const int ws = cap_waves(P->kind, P->max_m_short, cand[k]);
+    float t = -1.0f, tl = -1.0f, ts = -1.0f;
+    if ((w != cand[k]) && (ws != cand[k])) { // both capped: the same launches as a smaller count
+      if (ms)
+        ms[k] = t;
+      continue;
+    }
+    P->waves = w;
+    P->waves_short = ws;
+    if (bytes) {
+      if ((rc = (vj_fint)hipMemcpyAsync(Gw, G, bytes, hipMemcpyDeviceToDevice, s)))
+        rc = set_error(VJ_EHIP - (int)rc, "autotune: copy failed");
+      if (!rc && cplx && (rc = (vj_fint)hipMemcpyAsync(Gwi, Gi, bytes, hipMemcpyDeviceToDevice, s)))
+        rc = set_error(VJ_EHIP - (int)rc, "autotune: copy failed");
+    }
+    if (!rc && (rc = (vj_fint)hipEventRecord(e0, s)))
+      rc = set_error(VJ_EHIP - (int)rc, "autotune: event failed");
+    if (!rc)
+      rc = run_any(P, Gw, Gwi, V, Vi, eS, fS, info, nullptr, s, em);
+    if (!rc && (rc = (vj_fint)hipEventRecord(e1, s)))
+      rc = set_error(VJ_EHIP - (int)rc, "autotune: event failed");
+    if (!rc && (rc = (vj_fint)hipEventSynchronize(e1)))
+      rc = set_error(VJ_EHIP - (int)rc, "autotune: synchronisation failed");
+    if (!rc && ((rc = (vj_fint)hipEventElapsedTime(&tl, e0, em)) || (rc = (vj_fint)hipEventElapsedTime(&ts, em, e1))))
+      rc = set_error(VJ_EHIP - (int)rc, "autotune: timing failed");
+    if (!rc) {
+      t = tl + ts;
+      // a capped count is the same launch as a smaller one: keep the first
+      if ((w == cand[k]) && (best_ms < 0.0f || tl < best_ms)) {
+        best_ms = tl;
         best = w;
+      }
+      if ((ws == cand[k]) && (best_ms_s < 0.0f || ts < best_ms_s)) {
+        best_ms_s = ts;
+        best_s = ws;
       }
     }
     if (ms)
       ms[k] = t;
   }
+  (void)hipEventDestroy(em);
+  P->waves_short = best_s;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   P->waves = best;
