@@ -145,16 +145,15 @@ def cpu_bjac2(kind: str, ins_np):
 
 
 # --------------------------------------------------------------------------- batched SVD
-def _bsvd_alg_bytes(m, n, info, transforms, swp, w):
-    """Algorithmic HBM bytes of one batched launch: every step reads each
-    column once (w m n) and a transformed pair writes its two G columns and
-    reads/writes its two V columns (w (2m + 4n)); a converged matrix runs
-    info + 1 sweeps of n - 1 steps (the last one without transformations)."""
+def _bsvd_alg_bytes(m, n, evals, transforms, w):
+    """Algorithmic HBM bytes of one batched run: every pair evaluation reads
+    the pair's two G columns (w 2m) and a transformed pair writes them and
+    reads/writes its two V columns (w (2m + 4n)).  evals counts the pair
+    evaluations the run made (a pair whose columns did not change since its
+    previous evaluation is not evaluated again)."""
     m = m.astype(np.float64)
     n = n.astype(np.float64)
-    sweeps = np.where(info < swp, info + 1, swp).astype(np.float64)
-    steps = sweeps * (n - 1)
-    return float(np.sum(steps * w * m * n + transforms.astype(np.float64) * w * (2 * m + 4 * n)))
+    return float(np.sum(evals.astype(np.float64) * w * 2 * m + transforms.astype(np.float64) * w * (2 * m + 4 * n)))
 
 
 def bench_bsvd(args, kind: str = "d"):
@@ -199,7 +198,8 @@ def bench_bsvd(args, kind: str = "d"):
         wall = time.perf_counter() - t0
         kms = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)]
         res = dict(plan=plan, wall=wall, kernel_ms=float(np.mean(kms)), info=info.cpu().numpy(),
-                   tr=tr.cpu().numpy(), G=G, eS=eS.cpu().numpy(), fS=fS.cpu().numpy(), waves=plan.waves, tune=tune)
+                   tr=tr.cpu().numpy(), G=G, eS=eS.cpu().numpy(), fS=fS.cpu().numpy(), waves=plan.waves, tune=tune,
+                   evals=plan.evals())
         return res
 
     w = 8 if kind == "d" else 4
@@ -207,7 +207,7 @@ def bench_bsvd(args, kind: str = "d"):
     info, tr = r["info"], r["tr"]
     if np.any(info < 0):
         raise RuntimeError(f"batched SVD failed on {int(np.sum(info < 0))} matrices")
-    alg = _bsvd_alg_bytes(m, npad, info, tr, 999, w)
+    alg = _bsvd_alg_bytes(m, npad, r["evals"], tr, w)
     k_s = r["kernel_ms"] / 1e3
     achieved = alg / k_s / 1e9
     value = nb * args.steps / r["wall"]
@@ -220,7 +220,9 @@ def bench_bsvd(args, kind: str = "d"):
                                "n ~ U{8..128} (odd n padded with a zero column), m ~ U{n..4n}, N(0,1) entries, "
                                "two persistent launches (m > 128 and m <= 128)",
                    "batch": nb, "waves_per_workgroup": r["waves"], "autotune_ms": r["tune"],
-                   "sweeps_mean": float(np.mean(info)), "sweeps_max": int(np.max(info))},
+                   "sweeps_mean": float(np.mean(info)), "sweeps_max": int(np.max(info)),
+                   "pair_evaluations": int(np.sum(r["evals"])),
+                   "pair_slots": int(np.sum((np.minimum(info, 998) + 1) * npad * (npad // 2)))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bsvd",
                      "kernel_avg_ms": r["kernel_ms"], "alg_bytes_per_launch": alg},

@@ -184,6 +184,11 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
 vj_fint vjh_bplan_run(void *plan, void *G, void *Gi, void *V, void *Vi, void *eS, void *fS, vj_fint *info,
                       uint64_t *transforms, void *stream);
 int vjh_bplan_waves(const void *plan);
+/* Pair evaluations of each matrix in the plan's last run (uint64_t evals[nb]).
+ * A pair whose columns did not change since its previous evaluation is not
+ * evaluated again, so this can be below sweeps x steps x n/2; it is what the
+ * run read.  Synchronises `stream`. */
+vj_fint vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream);
 /* Autotuner of a plan: runs it with 1, 2, 4 and 8 waves per workgroup (counts
  * that would spill registers are skipped) on a copy of the input G (Gi) made
  * into the scratch buffer Gw (Gwi), timing each of the two launches with HIP

@@ -278,3 +278,29 @@ def test_short_form_vs_plain_bitwise(kind):
             assert np.array_equal(Vid.cpu().numpy().reshape(-1), r["Vi"][offV[b]: offV[b] + nn * nn]), b
         assert np.array_equal(eS.cpu().numpy(), r["eS"][offS[b]: offS[b] + nn]), b
         assert np.array_equal(fS.cpu().numpy(), r["fS"][offS[b]: offS[b] + nn]), b
+
+
+def test_pair_evaluation_counts():
+    """vjh_bplan_evals: every matrix evaluates at most sweeps x n steps x n/2
+    pairs (fewer when unchanged pairs are not evaluated again), at least one
+    full sweep, and never fewer pairs than it transformed."""
+    import torch
+    m, n, npad, offG, offV, offS, tot, G = _batch(32, 919, nmax=64)
+    plan = BT.BatchedSVD(m, npad, kind="d")
+    Gd = torch.from_numpy(G.copy()).cuda()
+    V = torch.zeros(tot[1], dtype=torch.float64, device="cuda")
+    eS = torch.zeros(tot[2], dtype=torch.float64, device="cuda")
+    fS = torch.zeros(tot[2], dtype=torch.float64, device="cuda")
+    info = torch.zeros(len(m), dtype=torch.int64, device="cuda")
+    tr = torch.zeros(len(m), dtype=torch.int64, device="cuda")
+    print("batched evals", flush=True)
+    plan.run(Gd, V, eS, fS, info, tr)
+    torch.cuda.synchronize()
+    ev = plan.evals().astype(np.int64)
+    plan.close()
+    info, tr = info.cpu().numpy(), tr.cpu().numpy()
+    nn = npad.astype(np.int64)
+    slots = (info + 1) * nn * (nn // 2)
+    assert np.all(info > 0)
+    assert np.all(ev <= slots) and np.all(ev >= nn * (nn // 2)) and np.all(ev >= tr)
+    print(f"evaluated {ev.sum()} of {slots.sum()} pair slots")

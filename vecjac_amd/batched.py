@@ -30,6 +30,8 @@ def _declare(L):
     L.vjh_bplan_destroy.argtypes = [C.c_void_p]
     L.vjh_bplan_waves.restype = C.c_int
     L.vjh_bplan_waves.argtypes = [C.c_void_p]
+    L.vjh_bplan_evals.restype = C.c_int64
+    L.vjh_bplan_evals.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.vjh_bplan_autotune.restype = C.c_int64
     L.vjh_bplan_autotune.argtypes = [C.c_void_p] + [C.c_void_p] * 9 + [C.POINTER(C.c_float), C.c_void_p]
     L._vj_batched_declared = True
@@ -98,6 +100,14 @@ class BatchedSVD:
             raise RuntimeError(f"vjh_bplan_autotune failed ({rc}): {_lib.last_error()}")
         self.waves = L.vjh_bplan_waves(self.plan)
         return {w: float(ms[k]) for k, w in enumerate((1, 2, 4, 8)) if ms[k] >= 0}
+
+    def evals(self, stream=None) -> np.ndarray:
+        """Pair evaluations of each matrix in the last run (vjh_bplan_evals)."""
+        import torch
+        out = np.zeros(len(self.m), dtype=np.uint64)
+        s = (stream or torch.cuda.current_stream()).cuda_stream
+        _lib.check(_lib.lib().vjh_bplan_evals(self.plan, out.ctypes.data, s), "vjh_bplan_evals")
+        return out
 
     def close(self):
         if getattr(self, "plan", None):

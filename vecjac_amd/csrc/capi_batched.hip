@@ -31,6 +31,7 @@ struct BPlan {
   long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each)
   int *d_order = nullptr;
   unsigned *d_queue = nullptr; // work counters of the two launches
+  unsigned long long *d_evals = nullptr; // pair evaluations per matrix (last run)
 };
 
 __global__ void k_zero_u32(unsigned *p)
@@ -53,7 +54,7 @@ int cap_waves(int kind, int max_m, int waves)
 {
   const int e = (max_m <= 64) ? 1 : (max_m <= 128) ? 2 : (max_m <= 256) ? 4 : (max_m <= 512) ? 8 : 16;
   const int bytes = e * ((kind == 0 || kind == 1) ? 8 : 4) * ((kind & 1) ? 2 : 1);
-  const int limit = (kind & 1) ? 32 : 64;
+  const int limit = (kind == 1) ? 16 : ((kind == 3) ? 32 : 64);
   return (waves > 4 && bytes > limit) ? 4 : waves;
 }
 
@@ -79,6 +80,7 @@ int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, v
   a.fS = (T *)fS;
   a.info = (long long *)info;
   a.transforms = (unsigned long long *)transforms;
+  a.evals = P->d_evals;
   a.swp = P->swp;
   // a fill kernel, not hipMemsetAsync, so that the run can be graph-captured
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, P->d_queue);
@@ -175,6 +177,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_meta, std::max<size_t>(1, meta.size()) * sizeof(long long));
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_order, std::max<size_t>(1, order.size()) * sizeof(int));
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, 2 * sizeof(unsigned));
+  if (e == hipSuccess) e = hipMalloc((void **)&P->d_evals, std::max<size_t>(1, (size_t)nb) * sizeof(unsigned long long));
   if (e == hipSuccess && !meta.empty()) e = hipMemcpy(P->d_meta, meta.data(), meta.size() * sizeof(long long), hipMemcpyHostToDevice);
   if (e == hipSuccess && !order.empty()) e = hipMemcpy(P->d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -182,6 +185,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
     (void)hipFree(P->d_meta);
     (void)hipFree(P->d_order);
     (void)hipFree(P->d_queue);
+    (void)hipFree(P->d_evals);
     delete P;
     return nullptr;
   }
@@ -198,6 +202,18 @@ vj_fint vjh_bplan_run(void *plan, void *Gr, void *Gi, void *Vr, void *Vi, void *
 }
 
 int vjh_bplan_waves(const void *plan) { return plan ? ((const BPlan *)plan)->waves : 0; }
+
+vj_fint vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream)
+{
+  const BPlan *P = (const BPlan *)plan;
+  if (!P || !evals)
+    return set_error(-1, "null plan or output");
+  hipStream_t s = (hipStream_t)stream;
+  if (P->nb)
+    VJ_HIP(hipMemcpyAsync(evals, P->d_evals, (size_t)P->nb * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  VJ_HIP(hipStreamSynchronize(s));
+  return 0;
+}
 
 vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
                            void *fS, vj_fint *info, float *ms, void *stream)
@@ -277,6 +293,7 @@ void vjh_bplan_destroy(void *plan)
   (void)hipFree(P->d_meta);
   (void)hipFree(P->d_order);
   (void)hipFree(P->d_queue);
+  (void)hipFree(P->d_evals);
   delete P;
 }
 

@@ -260,14 +260,37 @@ template <typename T> __device__ __forceinline__ PairDec<T> lane_dec(const PairD
 template <typename T, bool CPLX, int E, int W, int GP, bool PF>
 __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n, const int nE,
                                           const unsigned st, const int wave, const int lane, const Step<T> &S, T &wG,
-                                          unsigned long long &wcnt, int &werr)
+                                          unsigned long long &wcnt, int &werr, int *mod, const int gstep,
+                                          unsigned long long &wev)
 {
   typedef typename Acc<T>::A A;
   constexpr int B = 64;
   const int npairs = n >> 1;
   if (wave >= npairs)
     return;
-  const int cnt = (npairs - wave + W - 1) / W;       // pairs of this wave
+  const int cnt0 = (npairs - wave + W - 1) / W;      // pair slots of this wave (slot i: pair wave + W i)
+  // A pair whose two columns were last modified before its previous
+  // evaluation, one sweep (n steps) ago, would repeat that evaluation's
+  // no-op (same columns, same decision: no transformation, the same norms),
+  // so it is not evaluated again unless a rescaling is pending.  Lane j of
+  // `slot` holds the j-th slot to evaluate.
+  int cnt = cnt0, slot = lane;
+  if ((S.sG == 0) && (S.sV == 0) && (gstep > n)) {
+    bool need = false;
+    if (lane < cnt0) {
+      unsigned pp, qq;
+      mod_pair((unsigned)n, st, (unsigned)(wave + W * lane), pp, qq);
+      need = (mod[pp] >= gstep - n) || (mod[qq] >= gstep - n);
+    }
+    const unsigned long long mk = __ballot(need);
+    cnt = __popcll(mk);
+    if (cnt == 0)
+      return;
+    const int below = __popcll(mk & ((1ull << lane) - 1ull));
+    const int pos = need ? below : (cnt + lane - below); // a permutation of the lanes
+    slot = __builtin_amdgcn_ds_permute(pos << 2, lane);
+  }
+  wev += (unsigned long long)cnt;
   const int ngr = (cnt + GP - 1) / GP;
   T xr[GP][E], yr[GP][E], xi[GP][E], yi[GP][E], vx[GP][2], vy[GP][2], vxi[GP][2], vyi[GP][2];
   unsigned p[GP], q[GP];
@@ -275,6 +298,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
   auto pair_of = [&](const int g, const int r, unsigned &pp, unsigned &qq) {
     int i = g * GP + r;
     i = (i < cnt) ? i : (cnt - 1);
+    i = __builtin_amdgcn_readlane(slot, i);
     mod_pair((unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
   };
   if constexpr (PF) {
@@ -366,8 +390,11 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
       int err = D.err;
       if (!err && (__ballot(!isfinite_(o.tG)) != 0ull))
         err = CPLX ? -630 : -580;
-      if (!err && ((D.code != 1) || S.sG))
+      if (!err && ((D.code != 1) || S.sG)) {
         store_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p[r], (int)q[r], lane, xr[r], yr[r], xi[r], yi[r]);
+        if (lane == 0)
+          mod[p[r]] = mod[q[r]] = gstep;
+      }
       if (S.sV || (!D.err && (D.code != 1) && (D.code != 3))) {
         if (S.sV)
           scale_vpair<T>(S.sV, vx[r], vy[r], vxi[r], vyi[r]);
@@ -640,7 +667,8 @@ template <typename T, bool CPLX, int E> struct GroupShape {
 template <typename T, bool CPLX, int E, int W, bool PF>
 __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                            const int lg, const unsigned st, const int wave, const int lane,
-                                           const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr)
+                                           const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr,
+                                           unsigned long long &wev)
 {
   typedef typename Acc<T>::A A;
   const int LP = 1 << lg, P = 64 >> lg; // lanes per group, groups per wave
@@ -648,6 +676,7 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
   if (wave >= npairs)
     return;
   const int cnt = (npairs - wave + W - 1) / W; // pairs of this wave
+  wev += (unsigned long long)cnt;
   const int nit = (cnt + P - 1) >> (6 - lg);
   const int g = lane >> lg, t = lane & (LP - 1);
   const int nE = (m + LP - 1) >> lg, nEV = (n + LP - 1) >> lg;
@@ -806,16 +835,16 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
 template <typename T, bool CPLX, int E, int W>
 __device__ __forceinline__ void group_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                                const unsigned st, const int wave, const int lane, const Step<T> &S,
-                                               T &wG, unsigned long long &wcnt, int &werr)
+                                               T &wG, unsigned long long &wcnt, int &werr, unsigned long long &wev)
 {
   if ((E < 2) || CPLX || (m <= 64)) {
     group_step<T, CPLX, 4, W, GroupShape<T, CPLX, 4>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, 4, st, wave, lane, S, wG, wcnt,
-                                                          werr);
+                                                          werr, wev);
     return;
   }
   if constexpr ((E >= 2) && !CPLX)
     group_step<T, CPLX, 8, W, GroupShape<T, CPLX, 8>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, 4, st, wave, lane, S, wG, wcnt,
-                                                          werr);
+                                                          werr, wev);
 }
 
 // Longer columns: a pair per 64 lanes, two pairs per wave, with the register
@@ -823,32 +852,33 @@ __device__ __forceinline__ void group_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS
 template <typename T, bool CPLX, int E, int W>
 __device__ __forceinline__ void wave_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                               const unsigned st, const int wave, const int lane, const Step<T> &S,
-                                              T &wG, unsigned long long &wcnt, int &werr)
+                                              T &wG, unsigned long long &wcnt, int &werr, int *mod, const int gstep,
+                                              unsigned long long &wev)
 {
   const int nE = (m + 63) >> 6;
   if ((E < 4) || (nE <= 2)) {
     constexpr int E2 = (E >= 2) ? 2 : E;
     wave_step<T, CPLX, E2, W, StepShape<T, CPLX, E2>::GP, StepShape<T, CPLX, E2>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
-                                                                                      wave, lane, S, wG, wcnt, werr);
+                                                                                      wave, lane, S, wG, wcnt, werr, mod, gstep, wev);
     return;
   }
   if constexpr (E >= 4) {
     if ((E < 8) || (nE <= 4)) {
       wave_step<T, CPLX, 4, W, StepShape<T, CPLX, 4>::GP, StepShape<T, CPLX, 4>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
-                                                                                     wave, lane, S, wG, wcnt, werr);
+                                                                                     wave, lane, S, wG, wcnt, werr, mod, gstep, wev);
       return;
     }
   }
   if constexpr (E >= 8) {
     if ((E < 16) || (nE <= 8)) {
       wave_step<T, CPLX, 8, W, StepShape<T, CPLX, 8>::GP, StepShape<T, CPLX, 8>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
-                                                                                     wave, lane, S, wG, wcnt, werr);
+                                                                                     wave, lane, S, wG, wcnt, werr, mod, gstep, wev);
       return;
     }
   }
   if constexpr (E >= 16)
     wave_step<T, CPLX, E, W, StepShape<T, CPLX, E>::GP, StepShape<T, CPLX, E>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
-                                                                                   wave, lane, S, wG, wcnt, werr);
+                                                                                   wave, lane, S, wG, wcnt, werr, mod, gstep, wev);
 }
 
 // Minimum waves per SIMD the register allocation must allow: two, which keeps
@@ -856,7 +886,8 @@ __device__ __forceinline__ void wave_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS,
 // complex pairs, which would spill at 256 registers and run one wave per
 // SIMD instead.
 template <typename T, bool CPLX, int E> struct BsvdMinWaves {
-  static constexpr int value = (CPLX ? (E * (int)sizeof(T) >= 32) : (E * (int)sizeof(T) >= 128)) ? 1 : 2;
+  static constexpr int value =
+      (CPLX ? (E * (int)sizeof(T) * ((sizeof(T) == 8) ? 2 : 1) >= 32) : (E * (int)sizeof(T) >= 128)) ? 1 : 2;
 };
 
 template <typename T, bool CPLX, int E, int W, bool GRP>
@@ -864,6 +895,8 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
 {
   __shared__ int s_b;
   __shared__ T s_eS[128], s_fS[128];           // the matrix's (e,f) column norms (n <= 128)
+  __shared__ int s_mod[128];                   // step at which each column was last modified
+  __shared__ unsigned long long s_ev;          // pair evaluations of the matrix
   __shared__ double s_max[2][W][2];            // per-wave (tG, tV) maxima, by step parity
   __shared__ unsigned long long s_cnt[2][W];   // per-wave transformation counts, by step parity
   __shared__ int s_err[2][W];
@@ -887,6 +920,9 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     T *const eS = a.eS + a.offS[b];
     T *const fS = a.fS + a.offS[b];
     const unsigned stp = (unsigned)n; // steps per sweep of the modulus ordering
+    unsigned long long wev = 0ull;    // this wave's pair evaluations
+    if (threadIdx.x == 0)
+      s_ev = 0ull;
 
     // ---- setup (dvjsvd.c:78-184): max |G|, V := I, eS/fS, initial scale
     T mxv = (T)0;
@@ -918,6 +954,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       for (int j = threadIdx.x; j < n; j += 64 * W) {
         s_eS[j] = -inf_<T>();
         s_fS[j] = (T)1;
+        s_mod[j] = 0;
       }
     }
     const bool norms = !status; // eS/fS are written back (from LDS) at the end
@@ -946,17 +983,19 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       S.sG = 0;
       S.sV = 0;
       S.vtrack = 0;
-      int par = 0;
+      int par = 0, gstep = 0;
       for (sw = 0; sw < a.swp;) {
         unsigned long long swt = 0;
         for (unsigned st = 0; st < stp; ++st) {
+          ++gstep;
           T wG = (T)0, wV = (T)0;
           unsigned long long wcnt = 0;
           int werr = 0;
           if constexpr (GRP)
-            group_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr);
+            group_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr, wev);
           else
-            wave_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr);
+            wave_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr, s_mod,
+                                         gstep, wev);
           wG = red_max<T, 64>(wG, nullptr);
           if (lane == 0) {
             s_max[par][wave][0] = (double)wG;
@@ -1022,6 +1061,8 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           s_eS[j] -= (T)sT;
       }
     }
+    if (lane == 0 && wev)
+      atomicAdd(&s_ev, wev);
     __syncthreads();
     if (norms)
       for (int j = threadIdx.x; j < n; j += 64 * W) {
@@ -1032,6 +1073,8 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       a.info[b] = status ? (long long)status : (long long)sw;
       if (a.transforms)
         a.transforms[b] = total;
+      if (a.evals)
+        a.evals[b] = s_ev;
     }
   }
 }

@@ -14,6 +14,7 @@ template <typename T> struct BatchArgs {
   T *Gr, *Gi, *Vr, *Vi, *eS, *fS; // data (Gi/Vi null for real data)
   long long *info;                // per matrix: sweeps, or a negative code
   unsigned long long *transforms; // per matrix: transformations (may be null)
+  unsigned long long *evals;      // per matrix: pair evaluations (may be null)
   unsigned *queue;                // work counter, zero on entry
   unsigned swp;                   // sweep limit
 };
