@@ -33,20 +33,27 @@ struct PlainFlags {
   int status;
 };
 
+// The part of the control state that every workgroup of a step reads.  It
+// is double-buffered: the workgroups of a step read ctl[r] and fold the
+// previous step's slots into their own copy, and workgroup 0 stores that
+// copy in ctl[r ^ 1] for the next launch.
+template <typename T> struct PlainCtl {
+  T MG, MV;                      // running max-|.| bounds (dvjsvd.c:78, 88)
+  int sG, sV;                    // rescaling to apply during the next step
+};
+
 // Device-resident control block of one plain SVD.
 template <typename T> struct PlainState {
   int done;                      // converged, failed or out of sweeps
   int status;                    // 0 or the reference's negative code
   int zero;                      // G == 0: early return with 0 sweeps
   int ef_init;                   // inputs accepted: V/eS/fS initialised
-  T MG, MV;                      // running max-|.| bounds (dvjsvd.c:78, 88)
+  PlainCtl<T> ctl[2];
   unsigned long long MG_u, MV_u; // initial max-|.| (atomicMax on bit patterns)
   int NRM_EXP;                   // exponent of max/(2m) (dvjsvd.c:120-123)
   int sT, sV;                    // accumulated power-of-two shifts
   int sG_init, sV_init;          // initial scalings
-  int sG_next, sV_next;          // rescaling to apply during the next step
   unsigned sweeps;               // completed sweeps with transformations
-  unsigned blocks_done;          // last-workgroup detection
   unsigned long long swt;        // transformations in the current sweep
   unsigned long long hist[64];   // transformations per sweep (trace)
   // trace of the scalings (the reference's JTRACE lines, dvjsvd.c:70-635)
@@ -65,9 +72,9 @@ template <typename T> struct PlainState {
   double cos_max, sin_max, last_cos, last_sin;
 };
 
-// What one workgroup of a step reports to the step's last workgroup, which
-// reduces all of them: one slot per pair, written with device-scope atomic
-// stores (no contended atomics on shared counters).
+// What one workgroup of a step reports: one slot per pair.  The next launch
+// (or the end-of-sweep fold) reduces all slots of the step; the kernel
+// boundary orders the stores before those loads.
 struct PlainSlot {
   double tG, tV;                 // max |updated G / V element|
   double cosv, sinv;             // pair statistics (stats != 0)

@@ -5,9 +5,12 @@
 // (js[st][2k], js[st][2k+1]) and runs the whole step on it (norms, dot,
 // convergence test, 2x2 EVD, update of the G and V columns).  The control
 // state (max-|.| bounds, power-of-two shifts, per-sweep transformation
-// counter, convergence flag, error code) lives in device memory and is
-// updated by the last workgroup of each step, so sweeps run without any
-// host round trip; once the flag is set, later launches return at once.
+// counter, convergence flag, error code) lives in device memory.  Each
+// workgroup reports its pair in a slot; every workgroup of the next launch
+// reduces the step's slots while its own pair loads and folds them into the
+// control state (workgroup 0 stores the result), and a one-workgroup launch
+// folds the last step of each sweep.  Sweeps run without any host round
+// trip; once the convergence flag is set, later launches return at once.
 #include "svd_common.hpp"
 
 namespace vj {
@@ -44,11 +47,10 @@ __global__ void k_setup(const int m, PlainState<T> *st, const int vpreset)
   st->err_sw = st->err_st = 0u;
   const T MV = vpreset ? (T)__longlong_as_double((long long)st->MV_u) : (T)1;
   st->sT = st->sV = 0;
-  st->sG_next = st->sV_next = 0;
+  st->ctl[0].sG = st->ctl[0].sV = 0;
   st->sG_init = st->sV_init = 0;
   st->sweeps = 0;
   st->swt = 0;
-  st->blocks_done = 0u;
   st->zero = 0;
   st->status = 0;
   st->done = 0;
@@ -67,17 +69,17 @@ __global__ void k_setup(const int m, PlainState<T> *st, const int vpreset)
   st->eM0 = eM;
   st->sR0 = (CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R) - eM;
   st->NRM_EXP = NRM_EXP;
-  st->MG = ldexp_(MG, sN);
+  st->ctl[0].MG = ldexp_(MG, sN);
   st->sG_init = sN;
   st->sT = sN;
   int sV = 0;
   if (vpreset) {
     const int eV = frexp_exp(MV) - 1;
     sV = (CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R) - eV;
-    st->MV = ldexp_(MV, sV);
+    st->ctl[0].MV = ldexp_(MV, sV);
   }
   else
-    st->MV = (T)1;
+    st->ctl[0].MV = (T)1;
   st->sV_init = sV;
   st->sV = sV;
 }
@@ -171,6 +173,195 @@ __device__ __forceinline__ void store_rows(T *Gp, T *Gq, T *Gpi, T *Gqi, const i
   }
 }
 
+// Power-of-two scaling of E chunks held in registers (a rescaling decided
+// after the chunks were loaded).
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ void scale_rows(const int s, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E])
+{
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    xr[j] = ldexp_(xr[j], s);
+    yr[j] = ldexp_(yr[j], s);
+    if constexpr (CPLX) {
+      xi[j] = ldexp_(xi[j], s);
+      yi[j] = ldexp_(yi[j], s);
+    }
+  }
+}
+
+// --------------------------------------------------------- fold of a step
+// Maxima, transformation count and first failure over the slots of one
+// step, returned to every lane (maxima, an integer sum and a minimum do not
+// depend on the order of the reduction).
+struct SlotTotals {
+  double tG, tV, cosv, sinv;
+  unsigned long long trans;
+  unsigned long long key; // (pair index << 32 | code) of the first failing pair, ~0 if none
+};
+
+template <int B>
+__device__ __forceinline__ SlotTotals reduce_slots(const PlainSlot *__restrict__ sl, const int np, const int stats,
+                                                   const int lane, double *lds)
+{
+  SlotTotals t{0.0, 0.0, 0.0, 0.0, 0ull, ~0ull};
+  for (int i = lane; i < np; i += B) {
+    const int e = sl[i].err;
+    if (e) {
+      const unsigned long long key = ((unsigned long long)(unsigned)i << 32) | (unsigned)e;
+      t.key = (key < t.key) ? key : t.key;
+      continue;
+    }
+    t.tG = fmax(t.tG, sl[i].tG);
+    t.tV = fmax(t.tV, sl[i].tV);
+    t.trans += sl[i].trans;
+    if (stats) {
+      t.cosv = fmax(t.cosv, sl[i].cosv);
+      t.sinv = fmax(t.sinv, sl[i].sinv);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    t.tG = fmax(t.tG, __shfl_xor(t.tG, o, 64));
+    t.tV = fmax(t.tV, __shfl_xor(t.tV, o, 64));
+    t.cosv = fmax(t.cosv, __shfl_xor(t.cosv, o, 64));
+    t.sinv = fmax(t.sinv, __shfl_xor(t.sinv, o, 64));
+    t.trans += __shfl_xor(t.trans, o, 64);
+    const unsigned long long k2 = __shfl_xor(t.key, o, 64);
+    t.key = (k2 < t.key) ? k2 : t.key;
+  }
+  if constexpr (B > 64) {
+    double *const w = lds + 6 * (lane >> 6);
+    if ((lane & 63) == 0) {
+      w[0] = t.tG; w[1] = t.tV; w[2] = t.cosv; w[3] = t.sinv;
+      w[4] = __longlong_as_double((long long)t.trans);
+      w[5] = __longlong_as_double((long long)t.key);
+    }
+    __syncthreads();
+    SlotTotals a{0.0, 0.0, 0.0, 0.0, 0ull, ~0ull};
+    for (int w2 = 0; w2 < B / 64; ++w2) {
+      const double *const u = lds + 6 * w2;
+      a.tG = fmax(a.tG, u[0]); a.tV = fmax(a.tV, u[1]); a.cosv = fmax(a.cosv, u[2]); a.sinv = fmax(a.sinv, u[3]);
+      a.trans += (unsigned long long)__double_as_longlong(u[4]);
+      const unsigned long long k2 = (unsigned long long)__double_as_longlong(u[5]);
+      a.key = (k2 < a.key) ? k2 : a.key;
+    }
+    __syncthreads(); // lds is reused by the step
+    t = a;
+  }
+  return t;
+}
+
+template <typename T> struct Fold {
+  PlainCtl<T> c;  // the state before the next step
+  int eM, sR;     // exponent of MG and the rotation headroom (trace)
+  bool shiftG;    // a rescaling of G was decided
+};
+
+// dvjsvd.c:198-233: fold a step's maxima into the running bounds and decide
+// the power-of-two rescaling of G and V for the next step.
+template <typename T, bool CPLX>
+__device__ __forceinline__ Fold<T> fold_ctl(const PlainCtl<T> &in, const SlotTotals &t, const int NRM_EXP)
+{
+  const T nMG = (T)t.tG, nMV = (T)t.tV;
+  T MG = (nMG > in.MG) ? nMG : in.MG;
+  T MV = (nMV > in.MV) ? nMV : in.MV;
+  const int rot = CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R;
+  Fold<T> f;
+  f.eM = frexp_exp(MG) - 1;
+  f.sR = rot - f.eM;
+  f.shiftG = (f.sR < 0);
+  int sG = 0, sV = 0;
+  if (f.shiftG) {
+    sG = NRM_EXP - f.eM - 1;
+    MG = ldexp_(MG, sG);
+  }
+  const int eV = frexp_exp(MV) - 1;
+  if (rot - eV < 0) {
+    sV = rot - eV;
+    MV = ldexp_(MV, sV);
+  }
+  f.c.MG = MG;
+  f.c.MV = MV;
+  f.c.sG = sG;
+  f.c.sV = sV;
+  return f;
+}
+
+// The stores of the fold of step `folded` (one lane of one workgroup): the
+// next ctl, the shift totals and their trace, the sweep's transformation
+// count and, after the last step of a sweep, the end-of-sweep test
+// (dvjsvd.c:599-601).  A failing pair ends the run with its code.
+template <typename T>
+__device__ void fold_commit(PlainState<T> *st, const unsigned r, const SlotTotals &t, const Fold<T> &f,
+                            const unsigned folded, const unsigned stp, const unsigned swp, const int stats)
+{
+  if (t.key != ~0ull) {
+    st->status = (int)(unsigned)(t.key & 0xffffffffull);
+    st->err_sw = st->sweeps;
+    st->err_st = folded;
+    st->ctl[r ^ 1u] = st->ctl[r];
+    st->done = 1;
+    return;
+  }
+  st->swt += t.trans;
+  if (stats) {
+    st->cos_max = fmax(st->cos_max, t.cosv);
+    st->sin_max = fmax(st->sin_max, t.sinv);
+  }
+  const bool wrap = (folded + 1u == stp);
+  if (f.shiftG) {
+    st->sT += f.c.sG;
+    if (st->nev < 32u) { // logged against the step it precedes
+      auto &e = st->ev[st->nev++];
+      e.sw = st->sweeps + (wrap ? 1u : 0u);
+      e.st = wrap ? 0u : folded + 1u;
+      e.eM = f.eM;
+      e.sR = f.sR;
+      e.sN = f.c.sG;
+      e.M = f.c.MG;
+    }
+  }
+  st->sV += f.c.sV;
+  st->ctl[r ^ 1u] = f.c;
+  if (wrap) {
+    if (stats) {
+      st->last_cos = st->cos_max;
+      st->last_sin = st->sin_max;
+      st->cos_max = st->sin_max = 0.0;
+    }
+    const unsigned long long swt = st->swt;
+    if (st->sweeps < 64u)
+      st->hist[st->sweeps] = swt;
+    if (!swt)
+      st->done = 1;
+    else {
+      st->sweeps += 1u;
+      st->swt = 0ull;
+      if (st->sweeps >= swp)
+        st->done = 1;
+    }
+  }
+}
+
+// The fold of the last step of a sweep, in a launch of its own (one
+// workgroup), so that the convergence flag is current when the sweep ends.
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(256) void k_fold(PlainState<T> *__restrict__ st, const PlainSlot *__restrict__ prev,
+                                              const int np, const unsigned r, const unsigned stp, const unsigned swp,
+                                              const int stats)
+{
+  __shared__ double lds[6 * 4];
+  const int lane = threadIdx.x;
+  if (*(volatile int *)&st->done) {
+    if (lane == 0)
+      st->ctl[r ^ 1u] = st->ctl[r];
+    return;
+  }
+  const SlotTotals t = reduce_slots<256>(prev, np, stats, lane, lds);
+  if (lane == 0)
+    fold_commit<T>(st, r, t, fold_ctl<T, CPLX>(st->ctl[r], t, st->NRM_EXP), stp - 1u, stp, swp, stats);
+}
+
 // One step on pair blockIdx.x.  STREAM = false: the pair is held in
 // registers, E rows per lane.  STREAM = true (columns too long for the
 // registers): the pair is read three times in chunks of E rows per lane -
@@ -183,22 +374,23 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
                                              const unsigned *__restrict__ row, const unsigned st_idx,
                                              const unsigned stp, const unsigned swp, const T tol, const T gst,
                                              const int vtrack, const int stats, PlainState<T> *__restrict__ st,
+                                             const unsigned r, const PlainSlot *__restrict__ prev,
                                              PlainSlot *__restrict__ slots)
 {
-  // n is the length of the V columns (the table row has gridDim.x pairs)
+  // n is the length of the V columns (the table row has gridDim.x pairs).
+  // The control state before this step is ctl[r] with, for st_idx > 0, the
+  // slots of step st_idx - 1 (prev) folded in; workgroup 0 stores the fold
+  // in ctl[r ^ 1].
   __shared__ double lds[8 * (B / 64) + 2];
-  __shared__ int last;
-  if (*(volatile int *)&st->done)
+  const int lane = threadIdx.x;
+  const bool fold = (st_idx > 0u);
+  if (*(volatile int *)&st->done) {
+    if (fold && (blockIdx.x == 0) && (lane == 0))
+      st->ctl[r ^ 1u] = st->ctl[r];
     return;
+  }
   const int k = blockIdx.x;
   const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
-  const int lane = threadIdx.x;
-  Step<T> S;
-  S.tol = tol;
-  S.gst = gst;
-  S.sG = st->sG_next;
-  S.sV = st->sV_next;
-  S.vtrack = vtrack;
 
   T *const Gp = Gr + (long long)p * ldG, *const Gq = Gr + (long long)q * ldG;
   T *const Gpi = CPLX ? Gi + (long long)p * ldG : nullptr, *const Gqi = CPLX ? Gi + (long long)q * ldG : nullptr;
@@ -208,6 +400,34 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   // latency overlaps the reductions) when it fits E chunks of B rows (E <= 8,
   // complex E <= 4)
   const bool vreg = !STREAM && (E * (CPLX ? 2 : 1) <= 8) && (n <= E * B);
+  T xr[E], yr[E], xi[E], yi[E], vx[E], vy[E], vxi[E], vyi[E];
+  // The pair (and the V pair) is requested before the previous step's slots
+  // are reduced, so that the two latencies overlap; a pending rescaling is
+  // applied once the fold has decided it.
+  if (!STREAM)
+    load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, 0, lane, xr, yr, xi, yi);
+  if (vreg)
+    load_rows<T, CPLX, B, E>(Vp, Vq, Vpi, Vqi, n, 0, 0, lane, vx, vy, vxi, vyi);
+  PlainCtl<T> c = st->ctl[r];
+  if (fold) {
+    const SlotTotals t = reduce_slots<B>(prev, (int)gridDim.x, stats, lane, lds);
+    const Fold<T> f = fold_ctl<T, CPLX>(c, t, st->NRM_EXP);
+    if ((k == 0) && (lane == 0))
+      fold_commit<T>(st, r, t, f, st_idx - 1u, stp, swp, stats);
+    if (t.key != ~0ull)
+      return;
+    c = f.c;
+  }
+  Step<T> S;
+  S.tol = tol;
+  S.gst = gst;
+  S.sG = c.sG;
+  S.sV = c.sV;
+  S.vtrack = vtrack;
+  if (!STREAM && S.sG)
+    scale_rows<T, CPLX, E>(S.sG, xr, yr, xi, yi);
+  if (vreg && S.sV)
+    scale_rows<T, CPLX, E>(S.sV, vx, vy, vxi, vyi);
   if (S.sV && !vreg) {
     for (int i = lane; i < n; i += B) {
       Vp[i] = ldexp_(Vp[i], S.sV);
@@ -222,10 +442,7 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   PairOut<T, CPLX> o;
   PairDec<T> D;
   if (vreg) {
-    T xr[E], yr[E], xi[E], yi[E], vx[E], vy[E], vxi[E], vyi[E];
     const int nEV = (n + B - 1) / B;
-    load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, S.sG, lane, xr, yr, xi, yi);
-    load_rows<T, CPLX, B, E>(Vp, Vq, Vpi, Vqi, n, 0, S.sV, lane, vx, vy, vxi, vyi);
     Step<T> Sg = S;
     Sg.vtrack = 0;
     o = pair_step<T, CPLX, B, E>(m, 0, xr, yr, xi, yi, nullptr, nullptr, nullptr, nullptr, eSp, fSp, eSq, fSq, Sg, lds,
@@ -245,8 +462,6 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
       store_rows<T, CPLX, B, E>(Vp, Vq, Vpi, Vqi, n, 0, lane, vx, vy, vxi, vyi);
   }
   else if (!STREAM) {
-    T xr[E], yr[E], xi[E], yi[E];
-    load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, S.sG, lane, xr, yr, xi, yi);
     o = pair_step<T, CPLX, B, E>(m, n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, eSp, fSp, eSq, fSq, S, lds, lane, D);
     if (!o.err && ((o.code != 1) || S.sG))
       store_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, lane, xr, yr, xi, yi);
@@ -254,7 +469,6 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   else {
     typedef typename Acc<T>::A A;
     const int nc = (m + B * E - 1) / (B * E); // chunks of E rows per lane
-    T xr[E], yr[E], xi[E], yi[E];
     // The sums are taken in the same pass as the maxima, scaled by a
     // predicted exponent: the norm exponent each column was left with (every
     // step evaluates every column), moved by a pending rescaling.  Power-of-
@@ -340,151 +554,26 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     }
   }
   if (lane == 0) {
-    // this pair's report, then the ticket; the wait makes the slot words
-    // complete before the ticket is taken
+    // this pair's report, read by the next launch
     PlainSlot *const sl = slots + blockIdx.x;
-    __hip_atomic_store(&sl->err, o.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sl->err = o.err;
     if (!o.err) {
       eS[p] = eSp; fS[p] = fSp;
       eS[q] = eSq; fS[q] = fSq;
-      __hip_atomic_store(&sl->tG, (double)o.tG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&sl->tV, (double)o.tV, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&sl->trans, (unsigned)o.trans, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sl->tG = (double)o.tG;
+      sl->tV = (double)o.tV;
+      sl->trans = (unsigned)o.trans;
       if (stats) {
         // |sin| of the rotation: c |tan|, the complex tangent being at + i sat
         T ta = vabs(D.at);
         if constexpr (CPLX)
           ta = sqrt_(fma_(D.sat, D.sat, ta * ta));
         const T sn = D.trans ? D.c * ta : (T)0;
-        __hip_atomic_store(&sl->cosv, (double)D.cosv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&sl->sinv, (double)sn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    last = (__hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u);
-  }
-  __syncthreads();
-  if (!last)
-    return;
-  // ---- last workgroup of the step: reduce the slots (all lanes), fold the
-  // maxima, decide the next step's rescaling (dvjsvd.c:198-233) and the end
-  // of the sweep (:599-601)
-  double rG = 0.0, rV = 0.0, rc = 0.0, rs = 0.0;
-  unsigned long long rt = 0ull, rkey = ~0ull; // first failing pair: (index, code)
-  for (int i = lane; i < (int)gridDim.x; i += B) {
-    const PlainSlot *const sl = slots + i;
-    const int e = __hip_atomic_load(&sl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (e) {
-      const unsigned long long key = ((unsigned long long)(unsigned)i << 32) | (unsigned)e;
-      rkey = (key < rkey) ? key : rkey;
-      continue;
-    }
-    rG = fmax(rG, __hip_atomic_load(&sl->tG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    rV = fmax(rV, __hip_atomic_load(&sl->tV, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    rt += __hip_atomic_load(&sl->trans, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (stats) {
-      rc = fmax(rc, __hip_atomic_load(&sl->cosv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      rs = fmax(rs, __hip_atomic_load(&sl->sinv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    }
-  }
-#pragma unroll
-  for (int o2 = 32; o2 >= 1; o2 >>= 1) {
-    rG = fmax(rG, __shfl_xor(rG, o2, 64));
-    rV = fmax(rV, __shfl_xor(rV, o2, 64));
-    rc = fmax(rc, __shfl_xor(rc, o2, 64));
-    rs = fmax(rs, __shfl_xor(rs, o2, 64));
-    rt += __shfl_xor(rt, o2, 64);
-    const unsigned long long k2 = __shfl_xor(rkey, o2, 64);
-    rkey = (k2 < rkey) ? k2 : rkey;
-  }
-  if constexpr (B > 64) {
-    double *const w = lds + 6 * (lane >> 6);
-    if ((lane & 63) == 0) {
-      w[0] = rG; w[1] = rV; w[2] = rc; w[3] = rs;
-      w[4] = __longlong_as_double((long long)rt);
-      w[5] = __longlong_as_double((long long)rkey);
-    }
-    __syncthreads();
-    for (int w2 = 1; w2 < B / 64; ++w2) {
-      const double *const u = lds + 6 * w2;
-      rG = fmax(rG, u[0]); rV = fmax(rV, u[1]); rc = fmax(rc, u[2]); rs = fmax(rs, u[3]);
-      rt += (unsigned long long)__double_as_longlong(u[4]);
-      const unsigned long long k2 = (unsigned long long)__double_as_longlong(u[5]);
-      rkey = (k2 < rkey) ? k2 : rkey;
-    }
-  }
-  if (lane)
-    return;
-  st->blocks_done = 0u;
-  if (rkey != ~0ull) {
-    st->status = (int)(unsigned)(rkey & 0xffffffffull);
-    st->err_sw = st->sweeps;
-    st->err_st = st_idx;
-    st->done = 1;
-    return;
-  }
-  const T nMG = (T)rG, nMV = (T)rV;
-  st->swt += rt;
-  if (stats) {
-    st->cos_max = fmax(st->cos_max, rc);
-    st->sin_max = fmax(st->sin_max, rs);
-  }
-  // st->MG already carries this step's shift (applied when it was decided)
-  T MG = st->MG;
-  MG = (nMG > MG) ? nMG : MG;
-  T MV = st->MV;
-  MV = (nMV > MV) ? nMV : MV;
-  const int rot = CPLX ? Lim<T>::ROT_EXP_C - 1 : Lim<T>::ROT_EXP_R;
-  int sG = 0, sV = 0;
-  {
-    const int eM = frexp_exp(MG) - 1;
-    if (rot - eM < 0) {
-      sG = st->NRM_EXP - eM - 1;
-      MG = ldexp_(MG, sG);
-      st->sT += sG;
-      if (st->nev < 32u) {  // logged against the step it precedes
-        const bool wrap = (st_idx + 1u == stp);
-        auto &e = st->ev[st->nev++];
-        e.sw = st->sweeps + (wrap ? 1u : 0u);
-        e.st = wrap ? 0u : st_idx + 1u;
-        e.eM = eM;
-        e.sR = rot - eM;
-        e.sN = sG;
-        e.M = MG;
+        sl->cosv = (double)D.cosv;
+        sl->sinv = (double)sn;
       }
     }
   }
-  {
-    const int eV = frexp_exp(MV) - 1;
-    if (rot - eV < 0) {
-      sV = rot - eV;
-      MV = ldexp_(MV, sV);
-      st->sV += sV;
-    }
-  }
-  st->MG = MG;
-  st->MV = MV;
-  st->sG_next = sG;
-  st->sV_next = sV;
-  if (st_idx == stp - 1u) {
-    if (stats) {
-      st->last_cos = st->cos_max;
-      st->last_sin = st->sin_max;
-      st->cos_max = st->sin_max = 0.0;
-    }
-    const unsigned long long swt = st->swt;
-    if (st->sweeps < 64u)
-      st->hist[st->sweeps] = swt;
-    if (!swt)
-      st->done = 1;
-    else {
-      st->sweeps += 1u;
-      st->swt = 0ull;
-      if (st->sweeps >= swp)
-        st->done = 1;
-    }
-  }
-  __threadfence();
 }
 
 // ----------------------------------------------------------------- epilogue
@@ -519,18 +608,18 @@ __global__ __launch_bounds__(256) void k_epilogue(const int m, const int n, T *_
 
 // ------------------------------------------------------------------ driver
 template <typename T, bool CPLX, int B, int E, bool STREAM = false>
-static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, PlainSlot *slots, const unsigned *row, unsigned s,
-                        const T tol, const T gst, hipStream_t strm)
+static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, unsigned r, const PlainSlot *prev, PlainSlot *slots,
+                        const unsigned *row, unsigned s, const T tol, const T gst, hipStream_t strm)
 {
   hipLaunchKernelGGL((k_step<T, CPLX, B, E, STREAM>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m,
                      (int)(a.nv ? a.nv : a.n), a.Gr, a.Gi, a.ldG, a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp,
-                     tol, gst, a.vpreset, a.stats, st, slots);
+                     tol, gst, a.vpreset, a.stats, st, r, prev, slots);
   return 0;
 }
 
 template <typename T, bool CPLX>
-using StepFn = int (*)(const PlainArgs<T> &, PlainState<T> *, PlainSlot *, const unsigned *, unsigned, T, T,
-                       hipStream_t);
+using StepFn = int (*)(const PlainArgs<T> &, PlainState<T> *, unsigned, const PlainSlot *, PlainSlot *,
+                       const unsigned *, unsigned, T, T, hipStream_t);
 
 template <typename T, bool CPLX>
 static StepFn<T, CPLX> pick_step(long long m, int &B_out)
@@ -565,13 +654,16 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   StepFn<T, CPLX> step = pick_step<T, CPLX>(a.m, B);
   if (!step)
     return set_error(-1, "m = %lld exceeds the largest supported column length", a.m);
-  // control block and per-pair slots, released in stream order on return
+  // control block and per-pair slots (two steps' worth: a step reads the
+  // previous step's slots while it writes its own), released in stream order
+  // on return
   Staging sg(s);
   PlainState<T> *st = nullptr;
   PlainSlot *slots = nullptr;
+  const size_t np = (size_t)(a.n / 2);
   if (int rc = sg.alloc(&st, 1))
     return rc;
-  if (int rc = sg.alloc(&slots, (size_t)(a.n / 2)))
+  if (int rc = sg.alloc(&slots, 2 * np))
     return rc;
   VJ_HIP(hipMemsetAsync(st, 0, sizeof(PlainState<T>), s));
   const int mx_blocks = 1024;
@@ -604,9 +696,18 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   if (run && (hipEventCreate(&tev[0]) != hipSuccess || hipEventCreate(&tev[1]) != hipSuccess ||
               hipEventRecord(tev[0], s) != hipSuccess))
     rc = set_error(VJ_EHIP, "event creation failed");
+  // ctl[r] is the control state before the next launch; each launch that
+  // folds a step moves it to ctl[r ^ 1]
+  unsigned r = 0;
   for (unsigned sw = 0; !rc && sw < a.swp; ++sw) {
-    for (unsigned k = 0; k < a.stp; ++k)
-      step(a, st, slots, a.js + (size_t)k * a.n, k, tol, gst, s);
+    for (unsigned k = 0; k < a.stp; ++k) {
+      step(a, st, r, slots + ((k + 1u) & 1u) * np, slots + (k & 1u) * np, a.js + (size_t)k * a.n, k, tol, gst, s);
+      if (k)
+        r ^= 1u;
+    }
+    hipLaunchKernelGGL((k_fold<T, CPLX>), dim3(1), dim3(256), 0, s, st, slots + ((a.stp - 1u) & 1u) * np, (int)np, r,
+                       a.stp, a.swp, a.stats);
+    r ^= 1u;
     if ((rc = check_launch()))
       break;
     ++launched;
@@ -656,7 +757,7 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
       run->hist[i] = h.hist[i];
     run->MG0 = (double)h.MG0;
     run->MG_scaled = (double)ldexp_host(h.MG0, h.sG_init);
-    run->MG_final = (double)h.MG;
+    run->MG_final = (double)h.ctl[r].MG;
     run->eM0 = h.eM0;
     run->sR0 = h.sR0;
     run->sN0 = h.sG_init;
@@ -676,8 +777,8 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
     run->steps_ms = steps_ms;
     run->last_cos = h.last_cos;
     run->last_sin = h.last_sin;
-    run->sG_pending = h.sG_next;
-    run->sV_pending = h.sV_next;
+    run->sG_pending = h.ctl[r].sG;
+    run->sV_pending = h.ctl[r].sV;
   }
   if (h.status)
     return h.status;
