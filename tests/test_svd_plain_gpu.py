@@ -109,7 +109,9 @@ def _checks(kind, A, Ai, r, tolmul=8.0):
     ("s", 100, 32), ("s", 700, 32),
     ("c", 80, 32), ("c", 1100, 16),
     # columns streamed in chunks (longer than the register-held pairs)
-    ("z", 5000, 16), ("d", 9000, 16), ("c", 4500, 8), ("s", 8500, 8),
+    ("z", 5000, 16), ("d", 9000, 16), ("c", 4500, 8), ("s", 8500, 8), ("z", 3000, 16), ("c", 2100, 8),
+    # one pair: a single step per sweep, folded by the end-of-sweep launch alone
+    ("d", 7, 2), ("z", 5, 2), ("d", 300, 2),
 ])
 def test_plain_vs_oracle(kind, m, n):
     A, Ai = _mat(kind, m, n, seed=m * 7 + n)

@@ -631,15 +631,18 @@ static StepFn<T, CPLX> pick_step(long long m, int &B_out)
   // lanes in chunks of 4 rows per lane, complex ones through 256 lanes in
   // chunks of 2 (116 registers, four workgroups per CU; on an 8192 x 2048
   // complex matrix 21% faster than 512 lanes with chunks of 4, which need
-  // 154 registers and fit one workgroup per CU)
+  // 154 registers and fit one workgroup per CU).  Complex columns of 2049 to
+  // 4096 rows stream too: in registers they need 228 per lane (one
+  // 512-lane workgroup per CU), 6% slower on 4096 x 1024; up to 2048 rows
+  // the registers win (by 23% on 2048 x 512).
   if (m <= 64) { B_out = 64; return &launch_steps<T, CPLX, 64, 1>; }
   if (m <= 128) { B_out = 64; return &launch_steps<T, CPLX, 64, 2>; }
   if (m <= 256) { B_out = 64; return &launch_steps<T, CPLX, 64, 4>; }
   if (m <= 512) { B_out = 64; return &launch_steps<T, CPLX, 64, 8>; }
   if (m <= 1024) { B_out = 256; return &launch_steps<T, CPLX, 256, 4>; }
   if (m <= 2048) { B_out = 256; return &launch_steps<T, CPLX, 256, 8>; }
-  if (m <= 4096) { B_out = 512; return &launch_steps<T, CPLX, 512, 8>; }
   if constexpr (!CPLX) {
+    if (m <= 4096) { B_out = 512; return &launch_steps<T, CPLX, 512, 8>; }
     if (m <= 8192) { B_out = 512; return &launch_steps<T, CPLX, 512, 16>; }
   }
   if constexpr (CPLX) {
