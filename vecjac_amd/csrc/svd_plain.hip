@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void k_fold(PlainState<T> *__restrict__ st, co
 {
   __shared__ double lds[6 * 4];
   const int lane = threadIdx.x;
-  if (*(volatile int *)&st->done) {
+  if (st->done) {
     if (lane == 0)
       st->ctl[r ^ 1u] = st->ctl[r];
     return;
@@ -384,13 +384,19 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   __shared__ double lds[8 * (B / 64) + 2];
   const int lane = threadIdx.x;
   const bool fold = (st_idx > 0u);
-  if (*(volatile int *)&st->done) {
+  // The pair and the flag are read together (one latency, not two).  A plain
+  // load of the flag is enough: it was set by an earlier launch, or, after a
+  // failing pair, by workgroup 0 of this one, and then every workgroup finds
+  // that failure in the slots itself.
+  const int k = blockIdx.x;
+  const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
+  const int done = st->done;
+  asm volatile("" ::"s"(p), "s"(q), "s"(done)); // all three loads issued before the test below
+  if (done) {
     if (fold && (blockIdx.x == 0) && (lane == 0))
       st->ctl[r ^ 1u] = st->ctl[r];
     return;
   }
-  const int k = blockIdx.x;
-  const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
 
   T *const Gp = Gr + (long long)p * ldG, *const Gq = Gr + (long long)q * ldG;
   T *const Gpi = CPLX ? Gi + (long long)p * ldG : nullptr, *const Gqi = CPLX ? Gi + (long long)q * ldG : nullptr;
