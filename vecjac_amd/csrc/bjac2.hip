@@ -1,5 +1,7 @@
 // Batched 2x2 Hermitian eigendecomposition on gfx950: one lane per problem,
-// FP64 (FP32) VALU, branch-free per lane, structure-of-arrays in HBM.
+// FP64 (FP32) VALU, structure-of-arrays in HBM.  For finite inputs all lanes
+// take one path (the special-value branches of scalef run only in waves that
+// hold an infinite or NaN input).
 //
 // Replaces dbjac2_/zbjac2_/sbjac2_/cbjac2_ (dbjac2.c:81-142, zbjac2.c:108-171)
 // and the internal dbjac2i/zbjac2i forms (dbjac2.c:178-251, zbjac2.c:222-299).
