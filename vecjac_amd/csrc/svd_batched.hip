@@ -651,11 +651,14 @@ __device__ __forceinline__ T grp_gs(T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[
   return tG;
 }
 
-// Registers the prefetched (double-buffered) form may use: the G and V
-// pairs of one group step are 4E (complex 8E) elements per lane.
+// Whether the lane-group form prefetches the next group's pairs.  It does
+// not: the double buffer takes the FP64 short-column kernel from 144 to 180
+// registers, i.e. from three waves per SIMD to two, and a wave of a matrix
+// narrower than 8W columns has a single group iteration, whose "next" pairs
+// would be the same columns loaded again.  Without it 4096 matrices of 64x32
+// take 11.9 ms instead of 13.1 ms, and 16384 of n <= 128 are not slower.
 template <typename T, bool CPLX, int E> struct GroupShape {
-  static constexpr int words = 4 * E * (CPLX ? 2 : 1) * (int)(sizeof(T) / 4);
-  static constexpr bool PF = (words <= 32);
+  static constexpr bool PF = false;
 };
 
 // One step of the strategy for the pairs of one wave (pairs wave, wave + W,
