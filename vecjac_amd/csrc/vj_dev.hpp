@@ -145,19 +145,49 @@ enum Jac2Form { JAC2_TAN = 0, JAC2_SIN = 1, JAC2_SEC = 2 };
 // Same operation sequence as D8JAC2_LOOP / D8JACL_LOOP / D8JACI_LOOP
 // (dbjac2.c:17-44, 51-78, 148-174): exact 2^es prescaling, tan(2phi) clipped
 // to sqrt(huge), tan(phi), cos, and eigenvalues unscaled by 2^-es.
+// BIG_EXP - getexp(x) of a finite x as an integer, CLAMP for a zero x.
+// getexp maps zero to -inf, so the reference's scale exponent min(e..., MAX)
+// is an integer of magnitude below CLAMP (at most BIG_EXP + 1074 in FP64,
+// BIG_EXP + 149 in FP32) when an input is nonzero, and MAX when all are
+// zero; scalef_fin clamps MAX to CLAMP.  The
+// integer minimum with zeros at CLAMP is therefore the exponent scalef_fin
+// applies, and ldexp by it (or by its negation) gives the same bits.
+template <typename T> __device__ __forceinline__ int jac2_ei(T x)
+{
+  return (x == (T)0) ? Lim<T>::CLAMP : ((int)Lim<T>::BIG_EXP + 1 - frexp_exp(x));
+}
+
+// scalef(x, y) for y = -e, e an integer exponent from jac2_ei (|e| <= CLAMP)
+template <typename T> __device__ __forceinline__ T jac2_unscale(T x, int e)
+{
+  return isnan_(x) ? (x + x) : ldexp_(x, -e);
+}
+
 template <typename T, int FORM>
 __device__ __forceinline__ void rjac2(T a11, T a22, T a21, T &c, T &at, T &l1, T &l2, bool &P)
 {
-  const T e1 = (T)Lim<T>::BIG_EXP - getexp(a11);
-  const T e2 = (T)Lim<T>::BIG_EXP - getexp(a22);
-  const T er = (T)Lim<T>::BIG_EXP - getexp(a21);
-  T es = xmin(xmin(e1, e2), xmin(er, (T)Lim<T>::MAX));
-  const T ar = scalef(a21, es);
-  const T a1 = scalef(a11, es);
-  const T a2 = scalef(a22, es);
+  // finite inputs (every lane but the special-value ones): integer exponents
+  const bool fin = isfinite_(a11) && isfinite_(a22) && isfinite_(a21);
+  int ei = 0;
+  T es = (T)0, ar, a1, a2;
+  if (fin) {
+    ei = xmin(xmin(jac2_ei(a11), jac2_ei(a22)), jac2_ei(a21));
+    ar = ldexp_(a21, ei);
+    a1 = ldexp_(a11, ei);
+    a2 = ldexp_(a22, ei);
+  }
+  else {
+    const T e1 = (T)Lim<T>::BIG_EXP - getexp(a11);
+    const T e2 = (T)Lim<T>::BIG_EXP - getexp(a22);
+    const T er = (T)Lim<T>::BIG_EXP - getexp(a21);
+    es = xmin(xmin(e1, e2), xmin(er, (T)Lim<T>::MAX));
+    ar = scalef(a21, es);
+    a1 = scalef(a11, es);
+    a2 = scalef(a22, es);
+    es = vneg(es);
+  }
   const T aa = vabs(ar);
   const T as = vsgn(ar);
-  es = vneg(es);
   const T ab = scalef(aa, (T)1);
   const T ad = a1 - a2;
   const T t2 = vor(xmin(xmax(ab / vabs(ad), (T)0), (T)Lim<T>::SQRT_HUGE), vsgn(ad));
@@ -178,8 +208,14 @@ __device__ __forceinline__ void rjac2(T a11, T a22, T a21, T &c, T &at, T &l1, T
   }
   const T L1 = fma_(t1, fma_(a2, t1, ab), a1) / s2;
   const T L2 = fma_(t1, fma_(a1, t1, -ab), a2) / s2;
-  l1 = scalef(L1, es);
-  l2 = scalef(L2, es);
+  if (fin) {
+    l1 = jac2_unscale(L1, ei);
+    l2 = jac2_unscale(L2, ei);
+  }
+  else {
+    l1 = scalef(L1, es);
+    l2 = scalef(L2, es);
+  }
   P = (L1 < L2);
 }
 
@@ -190,21 +226,34 @@ __device__ __forceinline__ void rjac2(T a11, T a22, T a21, T &c, T &at, T &l1, T
 template <typename T, int FORM>
 __device__ __forceinline__ void cjac2(T a11, T a22, T a21r, T a21i, T &c, T &cat, T &sat, T &l1, T &l2, bool &P)
 {
-  const T e1 = (T)Lim<T>::BIG_EXP - getexp(a11);
-  const T e2 = (T)Lim<T>::BIG_EXP - getexp(a22);
-  const T er = (T)Lim<T>::BIG_EXP - getexp(a21r);
-  const T ei = (T)Lim<T>::BIG_EXP - getexp(a21i);
-  T es = xmin((T)Lim<T>::MAX, xmin(xmin(e1, e2), xmin(er, ei)));
-  const T ar = scalef(a21r, es);
-  const T ai = scalef(a21i, es);
-  const T a1 = scalef(a11, es);
-  const T a2 = scalef(a22, es);
+  // finite inputs (every lane but the special-value ones): integer exponents
+  const bool fin = isfinite_(a11) && isfinite_(a22) && isfinite_(a21r) && isfinite_(a21i);
+  int ei = 0;
+  T es = (T)0, ar, ai, a1, a2;
+  if (fin) {
+    ei = xmin(xmin(jac2_ei(a11), jac2_ei(a22)), xmin(jac2_ei(a21r), jac2_ei(a21i)));
+    ar = ldexp_(a21r, ei);
+    ai = ldexp_(a21i, ei);
+    a1 = ldexp_(a11, ei);
+    a2 = ldexp_(a22, ei);
+  }
+  else {
+    const T e1 = (T)Lim<T>::BIG_EXP - getexp(a11);
+    const T e2 = (T)Lim<T>::BIG_EXP - getexp(a22);
+    const T er = (T)Lim<T>::BIG_EXP - getexp(a21r);
+    const T eim = (T)Lim<T>::BIG_EXP - getexp(a21i);
+    es = xmin((T)Lim<T>::MAX, xmin(xmin(e1, e2), xmin(er, eim)));
+    ar = scalef(a21r, es);
+    ai = scalef(a21i, es);
+    a1 = scalef(a11, es);
+    a2 = scalef(a22, es);
+    es = vneg(es);
+  }
   T ar_ = vabs(ar);
   T ai_ = vabs(ai);
   const T as = vsgn(ar);
   const T am = xmin(ar_, ai_);
   const T aM = xmax(ar_, ai_);
-  es = vneg(es);
   T aa = xmax(am / aM, (T)0);
   aa = sqrt_(fma_(aa, aa, (T)1)) * aM;
   ar_ = vor(xmin(ar_ / aa, (T)1), as);
@@ -232,8 +281,14 @@ __device__ __forceinline__ void cjac2(T a11, T a22, T a21r, T a21i, T &c, T &cat
   }
   const T L1 = fma_(t1, fma_(a2, t1, ab), a1) / s2;
   const T L2 = fma_(t1, fma_(a1, t1, -ab), a2) / s2;
-  l1 = scalef(L1, es);
-  l2 = scalef(L2, es);
+  if (fin) {
+    l1 = jac2_unscale(L1, ei);
+    l2 = jac2_unscale(L2, ei);
+  }
+  else {
+    l1 = scalef(L1, es);
+    l2 = scalef(L2, es);
+  }
   P = (L1 < L2);
 }
 
