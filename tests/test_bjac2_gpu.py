@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import vjo
-from tests.util import bit_mismatch, special_jac2
+from tests.util import bit_mismatch, nonfinite_jac2, special_jac2
 from vecjac_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -66,6 +66,38 @@ def test_bjac2_special_values(hip, kind):
         _, oo, po = vjo.bjac2(kind, ns, ins)
         for k, (a, b) in enumerate(zip(outs, oo)):
             assert bit_mismatch(a, b) == 0, f"output {k}"
+        assert np.array_equal(p, po)
+
+
+@pytest.mark.parametrize("kind", ["d", "z", "s", "c"])
+def test_bjac2_nonfinite(hip, kind):
+    """Lanes with an infinite or NaN input take the general scaling path,
+    the others the integer-exponent one, within the same waves.
+
+    Bit-exact except for one documented deviation: with an infinite input
+    the reference forms a NaN whose sign bit (set for x86's invalid-operation
+    NaN) reaches the tangent through sgn(a11 - a22); gfx950's NaNs do not
+    carry it, so a zero tangent (at, or cat/sat) may differ in its sign.
+    Only rows with a non-finite input, only those outputs and only zeros
+    may differ."""
+    dt = np.float64 if kind in "dz" else np.float32
+    ins = nonfinite_jac2(kind in "zc", dt)
+    n = ins[0].size
+    finite_row = np.all([np.isfinite(a) for a in ins], axis=0)
+    rot = (1,) if kind in "ds" else (1, 2)      # at; cat, sat
+    ui = np.uint64 if dt == np.float64 else np.uint32
+    for ns in (n, -n):
+        print(f"bjac2 nonfinite kind={kind} n={ns}", flush=True)
+        rc, outs, p = _run_dev(hip, kind, ns, ins)
+        _, oo, po = vjo.bjac2(kind, ns, ins)
+        assert rc == 0
+        for k, (a, b) in enumerate(zip(outs, oo)):
+            diff = (a.view(ui) != b.view(ui)) & ~(np.isnan(a) & np.isnan(b))
+            if k not in rot:
+                assert not diff.any(), f"output {k}"
+                continue
+            assert not (diff & finite_row).any(), f"output {k} on finite rows"
+            assert np.all((a[diff] == 0) & (b[diff] == 0)), f"output {k}: a difference other than the sign of a zero"
         assert np.array_equal(p, po)
 
 

@@ -43,3 +43,17 @@ def special_jac2(cplx: bool, dtype):
     with np.errstate(over="ignore"):
         cols = [arr[:, k].astype(dtype) for k in range(4)]
     return cols if cplx else cols[:3]
+
+
+def nonfinite_jac2(cplx: bool, dtype):
+    """2x2 inputs with infinities and NaNs among finite values (zeros,
+    subnormals, the largest finite): every combination of the diagonal and
+    the real part of a21, so that waves mix finite and non-finite lanes."""
+    f = np.finfo(dtype)
+    vals = [0.0, 1.0, -2.5, np.inf, -np.inf, np.nan, float(f.max), float(f.smallest_subnormal)]
+    rng = np.random.Generator(np.random.Philox(7))
+    rows = [(a, d, b, rng.choice(vals) if cplx else 0.0) for a in vals for d in vals for b in vals]
+    arr = np.array(rows, dtype=np.float64)
+    cols = [arr[:, k].astype(dtype) for k in range(4)]
+    return cols if cplx else cols[:3]
+
