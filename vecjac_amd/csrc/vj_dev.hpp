@@ -141,17 +141,13 @@ template <typename T> __device__ __forceinline__ void two_sum(T a, T b, T &s, T 
 // Form of the 2x2 outputs.
 enum Jac2Form { JAC2_TAN = 0, JAC2_SIN = 1, JAC2_SEC = 2 };
 
-// One real symmetric 2x2 problem [[a11, a21], [a21, a22]] per lane.
-// Same operation sequence as D8JAC2_LOOP / D8JACL_LOOP / D8JACI_LOOP
-// (dbjac2.c:17-44, 51-78, 148-174): exact 2^es prescaling, tan(2phi) clipped
-// to sqrt(huge), tan(phi), cos, and eigenvalues unscaled by 2^-es.
 // BIG_EXP - getexp(x) of a finite x as an integer, CLAMP for a zero x.
 // getexp maps zero to -inf, so the reference's scale exponent min(e..., MAX)
 // is an integer of magnitude below CLAMP (at most BIG_EXP + 1074 in FP64,
 // BIG_EXP + 149 in FP32) when an input is nonzero, and MAX when all are
-// zero; scalef_fin clamps MAX to CLAMP.  The
-// integer minimum with zeros at CLAMP is therefore the exponent scalef_fin
-// applies, and ldexp by it (or by its negation) gives the same bits.
+// zero; scalef_fin clamps MAX to CLAMP.  The integer minimum with zeros at
+// CLAMP is therefore the exponent scalef_fin applies, and ldexp by it (or by
+// its negation) gives the same bits.
 template <typename T> __device__ __forceinline__ int jac2_ei(T x)
 {
   return (x == (T)0) ? Lim<T>::CLAMP : ((int)Lim<T>::BIG_EXP + 1 - frexp_exp(x));
@@ -163,6 +159,10 @@ template <typename T> __device__ __forceinline__ T jac2_unscale(T x, int e)
   return isnan_(x) ? (x + x) : ldexp_(x, -e);
 }
 
+// One real symmetric 2x2 problem [[a11, a21], [a21, a22]] per lane.
+// Same operation sequence as D8JAC2_LOOP / D8JACL_LOOP / D8JACI_LOOP
+// (dbjac2.c:17-44, 51-78, 148-174): exact 2^es prescaling, tan(2phi) clipped
+// to sqrt(huge), tan(phi), cos, and eigenvalues unscaled by 2^-es.
 template <typename T, int FORM>
 __device__ __forceinline__ void rjac2(T a11, T a22, T a21, T &c, T &at, T &l1, T &l2, bool &P)
 {
