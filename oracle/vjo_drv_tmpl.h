@@ -11,7 +11,15 @@
  *     the w1/w2 workspace (dvjsvd.c:169-180, 402);
  *   - the workspace is allocated here; work[] and iwork[1..] are untouched.
  * strict=1 applies the reference's argument checks (multiples of VL,
- * 64-byte alignment, MXCSR); strict=0 accepts any even n and any m >= n. */
+ * 64-byte alignment, MXCSR); strict=0 accepts any even n and any m >= n.
+ * The norm, dot and apply loops over the pairs of a step run on a thread
+ * team with max/min reductions, as the reference's do (dvjsvd.c:244, 303,
+ * 463): the pairs of a step touch disjoint columns, so a successful run is
+ * bit-identical to a serial one.  Inside an enclosing parallel region (the
+ * batch helper) they run serially. */
+#ifndef VJO_PAR
+#define VJO_PAR(M, N) (((M) * (N)) >= 32768u)
+#endif
 
 #if CPLX
 #define DRVNAME CN(vjsvd_core)
@@ -250,6 +258,8 @@ vjo_fint DRVNAME(const int strict, const vjo_fnat *const m, const vjo_fnat *cons
       int overflow = 0;
       do {
         nMG = (T)0;
+        /* one thread per pair block, as the reference's loop (dvjsvd.c:244) */
+#pragma omp parallel for schedule(static) reduction(max:nMG) if(VJO_PAR(M, N))
         for (vjo_fnat pq = 0u; pq < N; pq += 2u) {
           const vjo_fnat k = (pq >> 1u);
           if (!(nMG <= (T)TMAX)) {
@@ -295,6 +305,7 @@ vjo_fint DRVNAME(const int strict, const vjo_fnat *const m, const vjo_fnat *cons
       } while (overflow);
       /* scaled dot-products (dvjsvd.c:297-331) */
       nMG = (T)0;
+#pragma omp parallel for schedule(static) reduction(min:nMG) if(VJO_PAR(M, N)) /* dvjsvd.c:303 */
       for (vjo_fnat pq = 0u; pq < N; pq += 2u) {
         const vjo_fnat k = (pq >> 1u);
         if (!(nMG >= (T)0)) {
@@ -387,6 +398,7 @@ vjo_fint DRVNAME(const int strict, const vjo_fnat *const m, const vjo_fnat *cons
       /* apply (dvjsvd.c:460-590) */
       nMG = (T)0;
       T nMV = (T)-0.0;
+#pragma omp parallel for schedule(static) reduction(max:nMG, nMV) if(VJO_PAR(M, N)) /* dvjsvd.c:463 */
       for (vjo_fnat k = 0u; k < n_2; ++k) {
         const size_t p = r[2u * k], q = r[2u * k + 1u];
         stale[q] = stale[p] = (T)0;

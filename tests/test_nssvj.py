@@ -204,13 +204,18 @@ def test_sigma_vs_oracle(kind, m, n):
     if kind in "zc":
         Gi = vjo.aligned((n, m), RT[kind])
         Gi[...] = A.imag.T
-    with vjo.norm_mode(1):
-        rc, eS, fS, _, _, _ = vjo.vjsvd_core(kind, G, js=js, stp=stp, swp=999, Gi=Gi)
-    so = np.sort(vjo.sigma(eS, fS))[::-1]
-    assert info == 0 and rc >= 0 and abs(st[3] - (rc + 1)) <= 1
-    rel = np.max(np.abs(sva.astype(np.float64) * st[0] - so) / so)
-    print(f"{kind} m={m} n={n} max rel err vs oracle {rel:.3e}")
-    assert rel <= 8 * n * np.finfo(RT[kind]).eps
+    G0, Gi0 = G.copy(), (Gi.copy() if Gi is not None else None)
+    for mode in (1, 0):  # the GPU's norm and dot; the reference's own
+        G[...] = G0
+        if Gi is not None:
+            Gi[...] = Gi0
+        with vjo.norm_mode(mode):
+            rc, eS, fS, _, _, _ = vjo.vjsvd_core(kind, G, js=js, stp=stp, swp=999, Gi=Gi)
+        so = np.sort(vjo.sigma(eS, fS))[::-1]
+        assert info == 0 and rc >= 0 and abs(st[3] - (rc + 1)) <= 1
+        rel = np.max(np.abs(sva.astype(np.float64) * st[0] - so) / so)
+        print(f"{kind} m={m} n={n} mode {mode} max rel err vs oracle {rel:.3e}")
+        assert rel <= 8 * n * np.finfo(RT[kind]).eps
     _long_checks(kind, A, sva, st[0], U, V)
 
 

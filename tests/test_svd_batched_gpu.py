@@ -59,17 +59,18 @@ def test_batched_vs_oracle_f64(waves):
         mm, nn = int(m[b]), int(npad[b])
         A = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
         js, stp = S.pjs(4, nn)
-        Go = vjo.aligned((nn, mm), np.float64)
-        Go[...] = A
-        with vjo.norm_mode(1):
-            rc, eS, fS, V, _, st = vjo.vjsvd_core("d", Go, js=js, stp=stp, swp=999)
-        assert rc >= 0 and abs(int(r["info"][b]) - rc) <= 1, (b, r["info"][b], rc)
         sg = S.sigma(r["eS"][offS[b]: offS[b] + nn], r["fS"][offS[b]: offS[b] + nn])
-        so = S.sigma(eS, fS)
-        sg_s, so_s = np.sort(sg), np.sort(so)
-        nz = so_s > 0
-        assert np.max(np.abs(sg_s[nz] - so_s[nz]) / so_s[nz]) <= 8 * nn * EPS64
-        assert np.all(sg_s[~nz] == 0)
+        for mode in (1, 0):  # the GPU's norm and dot; the reference's own
+            Go = vjo.aligned((nn, mm), np.float64)
+            Go[...] = A
+            with vjo.norm_mode(mode):
+                rc, eS, fS, V, _, st = vjo.vjsvd_core("d", Go, js=js, stp=stp, swp=999)
+            assert rc >= 0 and abs(int(r["info"][b]) - rc) <= 1, (b, mode, r["info"][b], rc)
+            so = S.sigma(eS, fS)
+            sg_s, so_s = np.sort(sg), np.sort(so)
+            nz = so_s > 0
+            assert np.max(np.abs(sg_s[nz] - so_s[nz]) / so_s[nz]) <= 8 * nn * EPS64, (b, mode)
+            assert np.all(sg_s[~nz] == 0)
         U = r["G"][offG[b]: offG[b] + mm * nn].reshape(nn, mm)[sg > 0].astype(np.longdouble)
         Vb = r["V"][offV[b]: offV[b] + nn * nn].reshape(nn, nn).astype(np.longdouble)
         assert np.max(np.abs(U @ U.T - np.eye(U.shape[0], dtype=np.longdouble))) <= 8 * nn * EPS64

@@ -104,3 +104,21 @@ def test_grad_vs_mpmath_finite_differences():
         err = abs(an - float(fd))
         print(f"directional derivative {an:.12e} FD {float(fd):.12e} err {err:.2e}")
         assert err <= 1e-8 * max(1.0, abs(an))
+
+
+def test_failed_member_raises():
+    """A member with an infinite entry fails in the forward driver (-80,
+    dvjsvd.c:78-82): jacobi_svd raises and names it, instead of returning
+    unwritten singular values; JacobiSVD.apply returns the per-member codes
+    so that a caller can mask the member instead."""
+    import torch
+    from vecjac_amd.autograd import JacobiSVD, jacobi_svd
+    A, _, _, _ = W.grad_batch(6, 32, 16, 13)
+    A[4, 3, 5] = np.inf
+    At = torch.from_numpy(A).cuda()
+    print("jacobi_svd with an infinite entry in member 4", flush=True)
+    with pytest.raises(RuntimeError, match=r"members \[4\]"):
+        jacobi_svd(At)
+    _, _, _, info = JacobiSVD.apply(At)
+    info = info.cpu().numpy()
+    assert info[4] == -80 and np.all(np.delete(info, 4) >= 0)

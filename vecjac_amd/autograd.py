@@ -3,6 +3,12 @@ hand-written HIP adjoint backward, wrapped in a torch.autograd.Function.
 
     U, S, V = jacobi_svd(A)      # A: (batch, m, n) float64 CUDA tensor, m >= n, n even
 
+A member whose SVD fails (a non-finite entry: the driver's -80/-92 codes,
+dvjsvd.c:78-82) or does not converge within the plan's sweep limit makes
+jacobi_svd raise a RuntimeError that names the members and their codes;
+JacobiSVD.apply(A) returns the per-member codes as a fourth output instead,
+for callers that mask failed members themselves.
+
 The outputs are the thin SVD A = U diag(S) V^T with S sorted in decreasing
 order and each singular-vector pair normalised so that the entry of largest
 magnitude of every U column is positive (a fixed gauge, so that losses of U
@@ -26,12 +32,15 @@ _PLANS: dict = {}
 AUTOTUNE = True
 
 
+SWEEP_LIMIT = 999
+
+
 def _plan(B: int, m: int, n: int, kind: str):
     """The cached plan of this shape and whether it was just created."""
     key = (B, m, n, kind, torch.cuda.current_device())
     p = _PLANS.get(key)
     if p is None:
-        p = BT.BatchedSVD([m] * B, [n] * B, kind=kind)
+        p = BT.BatchedSVD([m] * B, [n] * B, kind=kind, swp=SWEEP_LIMIT)
         _PLANS[key] = p
         return p, True
     return p, False
@@ -104,9 +113,23 @@ class JacobiSVD(torch.autograd.Function):
         return svd_backward(U, S, V, gU, gS, gV)
 
 
+def check_info(info: torch.Tensor, swp: int = SWEEP_LIMIT) -> None:
+    """Raise if any member failed (negative code) or used all `swp` sweeps
+    without converging (code == swp, dvjsvd.c:604); one device sync."""
+    bad = ((info < 0) | (info >= swp)).nonzero().flatten()
+    if bad.numel():
+        idx = bad[:8].tolist()
+        codes = info[bad[:8]].tolist()
+        more = f" and {bad.numel() - len(idx)} more" if bad.numel() > len(idx) else ""
+        raise RuntimeError(f"jacobi_svd: members {idx}{more} failed or did not converge "
+                           f"(codes {codes}; negative = error, {swp} = sweep limit)")
+
+
 def jacobi_svd(A: torch.Tensor):
-    """(U, S, V) of a batch of matrices, differentiable (see module doc)."""
-    U, S, V, _ = JacobiSVD.apply(A)
+    """(U, S, V) of a batch of matrices, differentiable (see module doc).
+    Raises RuntimeError when a member fails or does not converge."""
+    U, S, V, info = JacobiSVD.apply(A)
+    check_info(info)
     return U, S, V
 
 
