@@ -164,6 +164,24 @@ vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, floa
                    float *eS, float *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
                    void *stream, unsigned *stats);
 
+/* Allocation-free twin of vjh_dvjsvd (no counterpart in the reference), for
+ * callers that cannot allocate device or pinned host memory: code that runs
+ * between graph captures, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, checks, results (bit for bit) and return codes
+ * as vjh_dvjsvd; it calls no hipMalloc/hipFree/hipHostMalloc and creates no
+ * event.  Its scratch (control block, table-check flag, per-pair slots) is a
+ * static device array of about 0.6 MiB per GPU, held until the process ends,
+ * which limits n to VJH_NOALLOC_MAX_N (larger n: -2 with a message).  Every
+ * call on one GPU shares that array and leaves its contents behind (each call
+ * clears what must start at zero); calls on one GPU are serialised by a
+ * per-GPU lock, since overlapping calls would race on the array.  It reads
+ * the convergence flag synchronously after each sweep (vjh_dvjsvd queues the
+ * sweeps one ahead), so it may be marginally slower.  Synchronises `stream`. */
+#define VJH_NOALLOC_MAX_N 16384
+vj_fint vjh_dvjsvd_noalloc(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS,
+                           double *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream,
+                           unsigned *stats);
+
 /* ---- 3. batched SVD and SVD adjoint (no reference counterpart) --------- */
 
 /* Plan for a batch of nb independent SVDs solved in two persistent launches

@@ -7,6 +7,7 @@
 #include "vecjac.h"
 
 #include <atomic>
+#include <mutex>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -251,9 +252,95 @@ int dev_vjsvd(vj_fnat m, vj_fnat n, T *Gr, T *Gi, vj_fnat ldG, T *Vr, T *Vi, vj_
   return rc;
 }
 
+// ------------------------------------------------------ allocation-free form
+// vjh_dvjsvd_noalloc keeps the scratch of one plain FP64 SVD (the control
+// block, the table-check flag and the 2 * (n/2) per-pair slots) in this
+// file-scope device array: one copy per GPU, held until the process ends.
+// In host code `scratch` names a host-side stand-in; the device address of
+// each GPU's copy comes from hipGetSymbolAddress on the first call on that
+// device and is used for every kernel argument and copy.
+constexpr size_t NA_STATE_OFF = 0, NA_FLAG_OFF = 8192, NA_SLOT_OFF = 8448;
+constexpr size_t NA_BYTES = NA_SLOT_OFF + (size_t)VJH_NOALLOC_MAX_N * sizeof(PlainSlot);
+static_assert(sizeof(PlainState<double>) <= NA_FLAG_OFF, "control block outgrew its scratch region");
+static_assert(NA_SLOT_OFF % alignof(PlainSlot) == 0, "slot region misaligned");
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEV = 64;
+std::mutex g_na_mu[NA_MAX_DEV];            // one call at a time per GPU
+unsigned char *g_na_ptr[NA_MAX_DEV] = {};  // device address of each GPU's scratch
+
+int noalloc_dvjsvd(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS, double *fS,
+                   const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats)
+{
+  if (!n)
+    return 0;
+  if (m < n) return -1;
+  if (n & 1u) return -2;
+  if (n > (vj_fnat)VJH_NOALLOC_MAX_N)
+    return set_error(-2, "n = %llu exceeds the %d columns the allocation-free scratch holds",
+                     (unsigned long long)n, VJH_NOALLOC_MAX_N);
+  if (ldG < m) return -4;
+  if (ldV < n) return -6;
+  int dev = -1;
+  VJ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NA_MAX_DEV)
+    return set_error(VJ_EHIP, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_na_mu[dev]);
+  if (!g_na_ptr[dev]) {
+    void *p = nullptr;
+    VJ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_ptr[dev] = (unsigned char *)p;
+  }
+  unsigned char *const base = g_na_ptr[dev];
+  hipStream_t s = (hipStream_t)stream;
+  if (stp) {
+    int *bad = (int *)(base + NA_FLAG_OFF);
+    if (int rc = zero_words(bad, sizeof(int), s))
+      return rc;
+    hipLaunchKernelGGL(k_table_check, dim3(stp), dim3(256), ((n + 31) / 32) * sizeof(unsigned), s, js, (unsigned)n, bad);
+    if (int rc = check_launch())
+      return rc;
+    int hbad = 0;
+    VJ_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    if (hbad)
+      return set_error(-9, "step table rows must be permutations of 0..n-1");
+  }
+  PlainArgs<double> a{};
+  a.m = (long long)m;
+  a.n = (long long)n;
+  a.Gr = G;
+  a.ldG = (long long)ldG;
+  a.Vr = V;
+  a.ldV = (long long)ldV;
+  a.eS = eS;
+  a.fS = fS;
+  a.js = js;
+  a.stp = stp;
+  a.swp = swp;
+  a.vpreset = (vpreset != 0u);
+  PlainWork<double> w{(PlainState<double> *)(base + NA_STATE_OFF), (PlainSlot *)(base + NA_SLOT_OFF)};
+  PlainRun run{};
+  const int rc = plain_svd<double, false>(a, &run, s, &w);
+  if (stats && rc > VJ_EHIP) {
+    stats[0] = run.sweeps;
+    stats[1] = run.launched_sweeps;
+    for (int i = 0; i < 64; ++i)
+      stats[2 + i] = (unsigned)run.hist[i];
+  }
+  return rc;
+}
+
 } // namespace
 
 extern "C" {
+
+vj_fint vjh_dvjsvd_noalloc(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS,
+                           double *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream,
+                           unsigned *stats)
+{
+  return noalloc_dvjsvd(m, n, G, ldG, V, ldV, eS, fS, js, stp, swp, vpreset, stream, stats);
+}
 
 void vjh_set_jtrace(int on) { g_jtrace.store(on ? 1 : 0); }
 

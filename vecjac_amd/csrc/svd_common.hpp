@@ -102,7 +102,22 @@ struct PlainRun {
   int sG_pending, sV_pending;    // shifts counted in sT / sV but not yet applied
 };
 
-template <typename T, bool CPLX> int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s);
+// Device workspace handed in by the caller (the allocation-free entry
+// point): the control block and 2 * (n/2) slots.  With it the driver makes no
+// allocation of its own: the control block is cleared by a fill kernel, and
+// the convergence flag is read back synchronously after each sweep (no
+// pinned host flags, no events), so the sweeps are not queued one ahead.
+template <typename T> struct PlainWork {
+  PlainState<T> *st;
+  PlainSlot *slots;
+};
+
+template <typename T, bool CPLX>
+int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s, const PlainWork<T> *work = nullptr);
+
+// Clears `bytes` (rounded up to whole 32-bit words) of device memory with a
+// fill kernel on stream s, in place of hipMemsetAsync.
+int zero_words(void *p, size_t bytes, hipStream_t s);
 
 // Host-side thresholds, evaluated exactly as the reference does
 // (dvjsvd.c:183-184): tol = sqrt(m) * eps/2, gst = tol * 2^FIN_EXP.

@@ -13,6 +13,8 @@
 // trip; once the convergence flag is set, later launches return at once.
 #include "svd_common.hpp"
 
+#include <algorithm>
+
 namespace vj {
 
 // ------------------------------------------------------------ setup kernels
@@ -660,8 +662,22 @@ static StepFn<T, CPLX> pick_step(long long m, int &B_out)
 
 template <typename T> static T ldexp_host(T x, int e) { return std::ldexp(x, e); }
 
+static __global__ void k_zero_words(unsigned *p, size_t words)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+int zero_words(void *p, size_t bytes, hipStream_t s)
+{
+  const size_t words = (bytes + 3u) / 4u;
+  const unsigned grid = (unsigned)std::min<size_t>((words + 255u) / 256u, 1024u);
+  hipLaunchKernelGGL(k_zero_words, dim3(grid ? grid : 1u), dim3(256), 0, s, (unsigned *)p, words);
+  return check_launch();
+}
+
 template <typename T, bool CPLX>
-int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
+int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s, const PlainWork<T> *work)
 {
   if (a.n == 0)
     return 0;
@@ -671,16 +687,24 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
     return set_error(-1, "m = %lld exceeds the largest supported column length", a.m);
   // control block and per-pair slots (two steps' worth: a step reads the
   // previous step's slots while it writes its own), released in stream order
-  // on return
+  // on return, or the caller's workspace; a step reads the previous step's
+  // slots only after the first step of a sweep, so the slots start unset
   Staging sg(s);
   PlainState<T> *st = nullptr;
   PlainSlot *slots = nullptr;
   const size_t np = (size_t)(a.n / 2);
-  if (int rc = sg.alloc(&st, 1))
+  if (work) {
+    st = work->st;
+    slots = work->slots;
+  }
+  else {
+    if (int rc = sg.alloc(&st, 1))
+      return rc;
+    if (int rc = sg.alloc(&slots, 2 * np))
+      return rc;
+  }
+  if (int rc = zero_words(st, sizeof(PlainState<T>), s))
     return rc;
-  if (int rc = sg.alloc(&slots, 2 * np))
-    return rc;
-  VJ_HIP(hipMemsetAsync(st, 0, sizeof(PlainState<T>), s));
   const int mx_blocks = 1024;
   hipLaunchKernelGGL(k_maxabs<T>, dim3(mx_blocks), dim3(256), 0, s, (int)a.m, (int)a.n, a.Gr, a.ldG, &st->MG_u);
   if (CPLX)
@@ -699,17 +723,21 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   const T tol = (a.tol > (T)0) ? a.tol : host_tol<T>(a.m);
   const T gst = host_gst<T>(tol);
   // Sweeps are enqueued one ahead of the convergence check: the host only
-  // reads the device flag of sweep k-1 after sweep k has been queued.
+  // reads the device flag of sweep k-1 after sweep k has been queued.  With a
+  // caller's workspace the flag of each sweep is read synchronously instead.
   PlainFlags *hf = nullptr;
-  VJ_HIP(hipHostMalloc((void **)&hf, 2 * sizeof(PlainFlags), hipHostMallocDefault));
-  hipEvent_t ev[2];
-  VJ_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  VJ_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (!work) {
+    VJ_HIP(hipHostMalloc((void **)&hf, 2 * sizeof(PlainFlags), hipHostMallocDefault));
+    VJ_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    VJ_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  }
   int rc = 0;
   unsigned launched = 0;
   hipEvent_t tev[2] = {nullptr, nullptr};
-  if (run && (hipEventCreate(&tev[0]) != hipSuccess || hipEventCreate(&tev[1]) != hipSuccess ||
-              hipEventRecord(tev[0], s) != hipSuccess))
+  const bool timed = run && !work;
+  if (timed && (hipEventCreate(&tev[0]) != hipSuccess || hipEventCreate(&tev[1]) != hipSuccess ||
+                hipEventRecord(tev[0], s) != hipSuccess))
     rc = set_error(VJ_EHIP, "event creation failed");
   // ctl[r] is the control state before the next launch; each launch that
   // folds a step moves it to ctl[r ^ 1]
@@ -726,6 +754,17 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
     if ((rc = check_launch()))
       break;
     ++launched;
+    if (work) {
+      PlainFlags f{};
+      if (hipMemcpyAsync(&f, &st->done, sizeof(PlainFlags), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = set_error(VJ_EHIP, "flag copy failed");
+        break;
+      }
+      if (f.done)
+        break;
+      continue;
+    }
     if (hipMemcpyAsync(&hf[sw & 1u], &st->done, sizeof(PlainFlags), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipEventRecord(ev[sw & 1u], s) != hipSuccess) {
       rc = set_error(VJ_EHIP, "flag copy failed");
@@ -745,7 +784,7 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
                        a.Vr, a.Vi, a.ldV, a.eS, a.fS, st, a.swp);
     rc = check_launch();
   }
-  if (!rc && run && hipEventRecord(tev[1], s) != hipSuccess)
+  if (!rc && timed && hipEventRecord(tev[1], s) != hipSuccess)
     rc = set_error(VJ_EHIP, "event record failed");
   PlainState<T> h;
   if (!rc) {
@@ -753,14 +792,16 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
       rc = set_error(VJ_EHIP, "state copy failed");
   }
   float steps_ms = 0.0f;
-  if (!rc && run && hipEventElapsedTime(&steps_ms, tev[0], tev[1]) != hipSuccess)
+  if (!rc && timed && hipEventElapsedTime(&steps_ms, tev[0], tev[1]) != hipSuccess)
     steps_ms = -1.0f;
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i) {
     if (tev[i])
       (void)hipEventDestroy(tev[i]);
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
-  (void)hipHostFree(hf);
+    if (ev[i])
+      (void)hipEventDestroy(ev[i]);
+  }
+  if (hf)
+    (void)hipHostFree(hf);
   if (rc)
     return rc;
   if (run) {
@@ -802,9 +843,9 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s)
   return (int)h.sweeps;
 }
 
-template int plain_svd<double, false>(const PlainArgs<double> &, PlainRun *, hipStream_t);
-template int plain_svd<double, true>(const PlainArgs<double> &, PlainRun *, hipStream_t);
-template int plain_svd<float, false>(const PlainArgs<float> &, PlainRun *, hipStream_t);
-template int plain_svd<float, true>(const PlainArgs<float> &, PlainRun *, hipStream_t);
+template int plain_svd<double, false>(const PlainArgs<double> &, PlainRun *, hipStream_t, const PlainWork<double> *);
+template int plain_svd<double, true>(const PlainArgs<double> &, PlainRun *, hipStream_t, const PlainWork<double> *);
+template int plain_svd<float, false>(const PlainArgs<float> &, PlainRun *, hipStream_t, const PlainWork<float> *);
+template int plain_svd<float, true>(const PlainArgs<float> &, PlainRun *, hipStream_t, const PlainWork<float> *);
 
 } // namespace vj
