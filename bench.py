@@ -565,12 +565,26 @@ def bench_grad(args):
     ev[0].record(stream)
     U, S, V, info = AG._forward(At)
     ev[1].record(stream)
-    gU, gS, gV = GUt.clone(), wt.clone(), GVt.clone()
-    ev[2].record(stream)
-    AG.svd_backward(U, S, V, gU, gS, gV)
-    ev[3].record(stream)
     torch.cuda.synchronize()
-    f_ms, b_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    f_ms = ev[0].elapsed_time(ev[1])
+    # the backward kernel alone: its operands already column-major per matrix
+    # (the wrapper's transposes are outside), averaged over the launches
+    from vecjac_amd import _lib
+    cm = lambda t: t.transpose(1, 2).contiguous()  # noqa: E731
+    Uc, Vc, gUc, gVc = cm(U), cm(V), cm(GUt), cm(GVt)
+    Sc, gSc = S.contiguous(), wt.contiguous()
+    dAk = torch.empty((nb, n, m), dtype=torch.float64, device="cuda")
+    L = _lib.lib()
+    bw = []
+    for _ in range(max(3, args.steps)):
+        ev[2].record(stream)
+        rc = L.vjh_dsvd_bwd(nb, m, n, Uc.data_ptr(), Sc.data_ptr(), Vc.data_ptr(), gUc.data_ptr(), gSc.data_ptr(),
+                            gVc.data_ptr(), dAk.data_ptr(), stream.cuda_stream)
+        ev[3].record(stream)
+        torch.cuda.synchronize()
+        assert rc == 0, _lib.last_error()
+        bw.append(ev[2].elapsed_time(ev[3]))
+    b_ms = float(np.mean(bw[1:]))
     # backward kernel bytes: reads U, dU (m n), V, dV (n^2), s, ds (n); writes dA (m n)
     bwd_bytes = nb * 8 * (3 * m * n + 2 * n * n + 2 * n)
     # the forward kernel alone (the cached plan's run, 90% of the step), with the
