@@ -46,7 +46,9 @@ extern "C" {
 /* ---- 1. reference-compatible entry points (host memory) ---------------- */
 
 /* Batched real symmetric 2x2 EVD; replaces dbjac2_ (dbjac2.h:6, dbjac2.c:81).
- * n<0: at receives sines; n>0: tangents.  p[i/8] bit i%8 = (l1[i] < l2[i]). */
+ * n<0: at receives sines; n>0: tangents.  p[i/8] bit i%8 = (l1[i] < l2[i]).
+ * Returns 1 on success, as the reference's release (OpenMP) build does
+ * (dbjac2.c:113, 132), -1 when |n| is not a multiple of 8 (16 for FP32). */
 vj_fint dbjac2_(const vj_fint *n, const double *a11, const double *a22, const double *a21,
                 double *c, double *at, double *l1, double *l2, unsigned *p);
 /* Batched Hermitian 2x2 EVD; replaces zbjac2_ (zbjac2.h:6, zbjac2.c:108). */
@@ -56,6 +58,20 @@ vj_fint zbjac2_(const vj_fint *n, const double *a11, const double *a22, const do
 vj_fint sbjac2_(const vj_fint *n, const float *a11, const float *a22, const float *a21,
                 float *c, float *at, float *l1, float *l2, unsigned *p);
 vj_fint cbjac2_(const vj_fint *n, const float *a11, const float *a22, const float *a21r, const float *a21i,
+                float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p);
+/* The internal forms; replace dbjac2i (dbjac2.h:9, dbjac2.c:178), zbjac2i
+ * (zbjac2.h:9, zbjac2.c:222), sbjac2i and cbjac2i: n<0 gives secants in at
+ * (cat, sat), n>=0 tangents; a block of 8 (FP32: 16) problems is solved only
+ * when the low 8 (16) bits of its word p[i/8] (p[i/16]) are not all zero,
+ * and those bits are then replaced by the block's l1 < l2 bits; a skipped
+ * block's outputs and word are left as they are.  p is read and written. */
+vj_fint dbjac2i(const vj_fint *n, const double *a11, const double *a22, const double *a21,
+                double *c, double *at, double *l1, double *l2, unsigned *p);
+vj_fint zbjac2i(const vj_fint *n, const double *a11, const double *a22, const double *a21r, const double *a21i,
+                double *c, double *cat, double *sat, double *l1, double *l2, unsigned *p);
+vj_fint sbjac2i(const vj_fint *n, const float *a11, const float *a22, const float *a21,
+                float *c, float *at, float *l1, float *l2, unsigned *p);
+vj_fint cbjac2i(const vj_fint *n, const float *a11, const float *a22, const float *a21r, const float *a21i,
                 float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p);
 
 /* One-sided Jacobi SVD of the m x n matrix G (m >= n), replacing dvjsvd_
@@ -72,7 +88,10 @@ vj_fint dvjsvd_(const vj_fnat *m, const vj_fnat *n, double *G, const vj_fnat *ld
  * on != 0 the host entry points ?vjsvd_ write the reference's trace file,
  * named by the string in `work`, in the reference's line format; the
  * per-phase times Tn..Te are 0 (one fused GPU step) and Tr is the GPU time
- * of the sweeps.  Process-wide; off by default. */
+ * of the sweeps.  Process-wide and sticky, as the compile-time switch is:
+ * every later host call, from any thread, reads `work` as a file name until
+ * it is switched off again, so a caller switches it on around the calls it
+ * traces only.  Off by default. */
 void vjh_set_jtrace(int on);
 
 /* Complex (split re/im) twin; replaces zvjsvd_ (zvjsvd.h:10, zvjsvd.c:40). */
@@ -136,7 +155,8 @@ void cnssvj_(const char *joba, const char *jobu, const char *jobv, const vj_fint
 
 /* ---- 2. device-memory extensions ---------------------------------------- */
 
-/* Same computations as ?bjac2_ on device arrays, enqueued on `stream`. */
+/* Same computations as ?bjac2_ (and, vjh_?bjac2i, as ?bjac2i) on device
+ * arrays, enqueued on `stream`; they return 0 on success. */
 vj_fint vjh_dbjac2(vj_fint n, const double *a11, const double *a22, const double *a21,
                    double *c, double *at, double *l1, double *l2, unsigned *p, void *stream);
 vj_fint vjh_zbjac2(vj_fint n, const double *a11, const double *a22, const double *a21r, const double *a21i,
@@ -145,6 +165,14 @@ vj_fint vjh_sbjac2(vj_fint n, const float *a11, const float *a22, const float *a
                    float *c, float *at, float *l1, float *l2, unsigned *p, void *stream);
 vj_fint vjh_cbjac2(vj_fint n, const float *a11, const float *a22, const float *a21r, const float *a21i,
                    float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream);
+vj_fint vjh_dbjac2i(vj_fint n, const double *a11, const double *a22, const double *a21,
+                    double *c, double *at, double *l1, double *l2, unsigned *p, void *stream);
+vj_fint vjh_zbjac2i(vj_fint n, const double *a11, const double *a22, const double *a21r, const double *a21i,
+                    double *c, double *cat, double *sat, double *l1, double *l2, unsigned *p, void *stream);
+vj_fint vjh_sbjac2i(vj_fint n, const float *a11, const float *a22, const float *a21,
+                    float *c, float *at, float *l1, float *l2, unsigned *p, void *stream);
+vj_fint vjh_cbjac2i(vj_fint n, const float *a11, const float *a22, const float *a21r, const float *a21i,
+                    float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream);
 
 /* Plain SVD on device memory: same driver as ?vjsvd_, with relaxed shape
  * rules (any even n <= m, any ld >= m, no alignment rule) and a device step
@@ -184,8 +212,11 @@ vj_fint vjh_dvjsvd_noalloc(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double 
 
 /* ---- 3. batched SVD and SVD adjoint (no reference counterpart) --------- */
 
-/* Plan for a batch of nb independent SVDs solved in two persistent launches
- * (matrices with m > 128, m > 64 complex, first; then the rest).
+/* Plan for a batch of nb independent SVDs solved in three persistent
+ * launches, one per class: matrices with m above the split (128, complex
+ * 64, by default), in the wave form; then the other ("short") ones in the
+ * lane-group form, first those whose G and V exceed the LDS budget (48 KiB
+ * by default), on global memory, then those within it, resident in LDS.
  * kind: 0 = FP64 real, 1 = FP64 complex (split re/im), 2 = FP32 real,
  * 3 = FP32 complex.  Host descriptors: m[b], n[b] (n even, 2 <= n <= 128,
  * n <= m <= 1024), element offsets offG[b] of G_b (ld m[b]), offV[b] of V_b (ld n[b])
@@ -194,26 +225,46 @@ vj_fint vjh_dvjsvd_noalloc(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double 
  * failure with *status < 0. */
 void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG,
                        const vj_fint *offV, const vj_fint *offS, unsigned swp, int waves, vj_fint *status);
-/* One run (the two launches) on device buffers: G (A in, U out; U columns of zero singular
+/* One run (the launches of the classes) on device buffers: G (A in, U out; U columns of zero singular
  * values are NaN as in the reference), V (out, starts as I), eS/fS (out),
  * info[b] (sweeps or a negative code), transforms[b] (rotations and
  * Gram-Schmidt steps of matrix b; may be NULL).  Gi/Vi are NULL for real
  * kinds.  Asynchronous on `stream`; graph-capturable. */
 vj_fint vjh_bplan_run(void *plan, void *G, void *Gi, void *V, void *Vi, void *eS, void *fS, vj_fint *info,
                       uint64_t *transforms, void *stream);
+/* Waves per workgroup of the first non-empty launch. */
 int vjh_bplan_waves(const void *plan);
+/* The plan's choices (out[12]): matrices per class (wave form, lane-group
+ * form on global memory, lane-group form in LDS), waves per workgroup of
+ * each launch, lanes per pair of each launch (64 for the wave form), the
+ * split row count, the LDS budget and the dynamic LDS bytes per workgroup of
+ * the LDS launch. */
+vj_fint vjh_bplan_config(const void *plan, vj_fint *out);
+/* Sets the plan's choices: the split row count (at most 128, complex 64),
+ * the LDS budget in bytes (at most 152 KiB; 0 keeps every matrix on global
+ * memory) and, where not NULL, waves[3] per workgroup of the three launches
+ * (1, 2, 4 or 8) and lanes[3] per pair of the two lane-group launches
+ * (lanes[1], lanes[2]: 16, 8 or 4; 8 lanes with at most 4 waves, 4 lanes
+ * with at most 2 waves and real rows up to 64).  Returns -5 for a shape that
+ * is not built.  The results do not depend on any of them. */
+vj_fint vjh_bplan_set(void *plan, int split_rows, vj_fint lds_budget, const int *waves, const int *lanes);
 /* Pair evaluations of each matrix in the plan's last run (uint64_t evals[nb]).
  * A pair whose columns did not change since its previous evaluation is not
  * evaluated again, so this can be below sweeps x steps x n/2; it is what the
  * run read.  Synchronises `stream`. */
 vj_fint vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream);
-/* Autotuner of a plan: runs it with 1, 2, 4 and 8 waves per workgroup (counts
- * that would spill registers are skipped) on a copy of the input G (Gi) made
- * into the scratch buffer Gw (Gwi), timing each of the two launches with HIP
- * events on `stream`, and keeps the fastest count of each launch for later
- * runs.  ms[4] (may be NULL) receives the run times, -1 for skipped counts.
- * V, eS, fS and info are overwritten.  Returns the count chosen for the
- * long-column launch, or a negative code.  Synchronises `stream`. */
+/* Autotuner of a plan, on copies of the input G (Gi) made into the scratch
+ * buffer Gw (Gwi), each launch timed on its own with HIP events on `stream`:
+ * (1) for the split row count and half of it, each launch at 1, 2, 4 and 8
+ * waves per workgroup (counts that would spill registers are skipped) and
+ * the lane-group launches also at 8 and 4 lanes per pair, keeping the split
+ * and shapes of the fastest total; (2) for LDS budgets of 0, 24, 48, 96 and
+ * 152 KiB, the two short launches at each shape, keeping the fastest
+ * budget and shapes.  The choices stay in the plan
+ * (vjh_bplan_config).  ms[4] (may be NULL) receives the run times of the
+ * four counts at the default split, -1 for skipped counts.  V, eS, fS and
+ * info are overwritten.  Returns the count of the first non-empty launch,
+ * or a negative code.  Synchronises `stream`. */
 vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
                            void *fS, vj_fint *info, float *ms, void *stream);
 void vjh_bplan_destroy(void *plan);

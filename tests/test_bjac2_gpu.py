@@ -107,7 +107,7 @@ def test_bjac2_host_entry(hip, kind):
     ins = [a for a in W.jac2_batch(n, 5, kind == "z") if a is not None]
     rc, outs, p = _run_host(hip, kind, -n, ins)
     _, oo, po = vjo.bjac2(kind, -n, ins)
-    assert rc == 0
+    assert rc == 1  # the reference's release (OpenMP) build returns 1 (dbjac2.c:113)
     for a, b in zip(outs, oo):
         assert bit_mismatch(a, b) == 0
     assert np.array_equal(p, po)
@@ -116,8 +116,60 @@ def test_bjac2_host_entry(hip, kind):
 def test_bjac2_bad_length(hip):
     # |n| not a multiple of the lane count: the reference's argument check
     rc, _, _ = _run_host(hip, "d", 8, [np.zeros(8)] * 3)
-    assert rc == 0
+    assert rc == 1
     import ctypes as C
     a = np.zeros(12)
     rc = hip.dbjac2_(C.byref(C.c_int64(12)), *([a.ctypes.data] * 7), np.zeros(2, np.uint32).ctypes.data)
     assert rc == -1
+
+
+@pytest.mark.parametrize("kind", ["d", "z", "s", "c"])
+@pytest.mark.parametrize("secants", [False, True])
+def test_bjac2_internal_forms(hip, kind, secants):
+    """dbjac2i/zbjac2i/sbjac2i/cbjac2i (dbjac2.c:178-251): n<0 secants, n>=0
+    tangents, a block of VL problems solved only when the low VL bits of its
+    p word are set; those bits become the block's l1 < l2 bits, the high bits
+    of p stay, and a skipped block keeps its outputs.  Device and host entry
+    points against the oracle's internal forms (bit-identical to the
+    reference build), with outputs pre-filled with a sentinel."""
+    import ctypes as C
+    import torch
+    dt = np.float64 if kind in "dz" else np.float32
+    vl = 8 if kind in "dz" else 16
+    n = 1 << 14
+    ins = [a for a in W.jac2_batch(n, 4321, kind in "zc", dtype=dt) if a is not None]
+    g = W.rng(11)
+    lo = (1 << vl) - 1
+    p0 = (g.integers(0, 1 << 16, n // vl).astype(np.uint32) << np.uint32(16)) | \
+        np.where(g.integers(0, 3, n // vl) == 0, 0, g.integers(1, lo + 1, n // vl)).astype(np.uint32)
+    ns = -n if secants else n
+    nout = 5 if kind in "zc" else 4
+    # oracle: outputs start as the sentinel too
+    sent = dt(-7.25)
+    rco, oo, po = vjo.bjac2(kind, ns, ins, internal=True, p_init=p0)
+    skipped = np.repeat((p0 & lo) == 0, vl)
+    for a in oo:
+        a[skipped] = sent
+    # device
+    dins = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in ins]
+    douts = [torch.full((n,), float(sent), dtype=torch.float64 if dt == np.float64 else torch.float32, device="cuda")
+             for _ in range(nout)]
+    dp = torch.from_numpy(p0.view(np.int32).copy()).cuda()
+    print(f"bjac2i kind={kind} n={ns}", flush=True)
+    rc = getattr(hip, f"vjh_{kind}bjac2i")(ns, *[t.data_ptr() for t in dins], *[t.data_ptr() for t in douts],
+                                            dp.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert rc == 0
+    for k, (a, b) in enumerate(zip(douts, oo)):
+        assert bit_mismatch(a.cpu().numpy(), b) == 0, f"device output {k}"
+    assert np.array_equal(dp.cpu().numpy().view(np.uint32), po)
+    # host
+    outs = [np.full(n, sent, dtype=dt) for _ in range(nout)]
+    ph = p0.copy()
+    hins = [np.ascontiguousarray(a, dtype=dt) for a in ins]
+    rc = getattr(hip, f"{kind}bjac2i")(C.byref(C.c_int64(ns)), *[a.ctypes.data for a in hins],
+                                        *[a.ctypes.data for a in outs], ph.ctypes.data)
+    assert rc == 1
+    for k, (a, b) in enumerate(zip(outs, oo)):
+        assert bit_mismatch(a, b) == 0, f"host output {k}"
+    assert np.array_equal(ph, po)

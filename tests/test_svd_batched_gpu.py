@@ -1,6 +1,7 @@
-"""GPU parity of the batched SVD (many uneven matrices in two persistent
-launches: columns longer than 128 rows (64 complex) in the wave form, the
-rest in the lane-group form).  Every matrix is checked against the oracle on
+"""GPU parity of the batched SVD (many uneven matrices in one persistent
+launch per class: columns longer than 128 rows (64 complex) in the wave
+form, the rest in the lane-group form, on global memory or, when a matrix's
+G and V fit the plan's LDS budget, resident in LDS).  Every matrix is checked against the oracle on
 the same input with the same step table (sigma within 8 n eps, long-double
 orthogonality and reconstruction) and against the plain GPU driver, which
 performs the same operations per pair and must agree bit for bit."""
@@ -25,10 +26,12 @@ def _batch(nb, seed, nmax=48, dtype=np.float64):
     return m, n, npad, offG, offV, offS, tot, G
 
 
-def _run(kind, m, npad, G, tot, waves=0, Gi=None):
+def _run(kind, m, npad, G, tot, waves=0, Gi=None, split=None, budget=None, wv=None, lanes=None):
     import torch
     dt = torch.float64 if kind in "dz" else torch.float32
     plan = BT.BatchedSVD(m, npad, kind=kind, waves=waves)
+    if budget is not None:
+        plan.set(split if split is not None else (64 if kind in "zc" else 128), budget, wv, lanes)
     Gd = torch.from_numpy(G.copy()).cuda()
     Gid = torch.from_numpy(Gi.copy()).cuda() if Gi is not None else None
     V = torch.zeros(tot[1], dtype=dt, device="cuda")
@@ -305,3 +308,39 @@ def test_pair_evaluation_counts():
     assert np.all(info > 0)
     assert np.all(ev <= slots) and np.all(ev >= nn * (nn // 2)) and np.all(ev >= tr)
     print(f"evaluated {ev.sum()} of {slots.sum()} pair slots")
+
+
+@pytest.mark.parametrize("kind", ["d", "z", "s", "c"])
+def test_lds_resident_form_bitwise(kind):
+    """The lane-group form resident in LDS computes the same bits as on
+    global memory, and so do its narrower groups (8 and 4 lanes per pair):
+    one batch (long members in the wave form, short ones of every width) run
+    with no LDS budget, with the largest one and with the default, at 16, 8
+    and 4 lanes per pair and several waves per workgroup; every output, the
+    sweep counts and the transformation counts agree.  Also at a split of
+    half the short-row bound, which moves members between the launches."""
+    dt = np.float64 if kind in "dz" else np.float32
+    m = np.array([8, 16, 40, 63, 64, 96, 127, 128, 200, 24, 48, 32, 64, 100], dtype=np.int64)
+    n = np.array([8, 10, 40, 30, 64, 34, 48, 64, 100, 24, 16, 32, 32, 90], dtype=np.int64)
+    if kind in "zc":
+        m = np.minimum(m, 64)
+        m = np.maximum(m, n + (n & 1))  # members wider than 64 stay in the wave form
+    _, npad, offG, offV, offS, tot = BT.pack_shapes(m, n)
+    G = BT.fill_gaussian(m, npad, n, offG, 771, dt)
+    Gi = BT.fill_gaussian(m, npad, n, offG, 772, dt) if kind in "zc" else None
+    sr = 64 if kind in "zc" else 128
+    runs = {}
+    shapes = [(sr, 0, None, None), (sr, 152 * 1024, None, None), (sr, None, None, None),
+              (sr // 2, 152 * 1024, None, None),
+              # 8 lanes per pair (8 pairs per wave), on global memory and in LDS
+              (sr, 0, (4, 2, 2), (16, 8, 8)), (sr, 152 * 1024, (4, 4, 1), (16, 8, 8))]
+    if kind in "ds":  # 4 lanes per pair (16 pairs per wave), real rows up to 64
+        shapes += [(64, 0, (4, 1, 2), (16, 4, 4)), (64, 152 * 1024, (4, 2, 1), (16, 4, 4))]
+    for split, budget, wv, lanes in shapes:
+        r = _run(kind, m, npad, G, tot, Gi=Gi, split=split, budget=budget, wv=wv, lanes=lanes)
+        assert r["rc"] == 0 and np.all(r["info"] > 0), (split, budget, r["info"])
+        runs[(split, budget, wv, lanes)] = r
+    ref = runs[(sr, 0, None, None)]
+    for key, r in runs.items():
+        for f in ("G", "V", "eS", "fS", "info", "tr") + (("Gi", "Vi") if kind in "zc" else ()):
+            assert np.array_equal(r[f], ref[f], equal_nan=True), (key, f)

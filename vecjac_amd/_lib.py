@@ -46,14 +46,14 @@ def lib() -> C.CDLL:
     L = C.CDLL(str(LIB_PATH))
     P = C.POINTER
     # reference-compatible 2x2 EVD (host arrays)
-    for name in ("dbjac2_", "sbjac2_"):
+    for name in ("dbjac2_", "sbjac2_", "dbjac2i", "sbjac2i"):
         _decl(L, name, _fint, [P(_fint), _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pu])
-    for name in ("zbjac2_", "cbjac2_"):
+    for name in ("zbjac2_", "cbjac2_", "zbjac2i", "cbjac2i"):
         _decl(L, name, _fint, [P(_fint), _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pu])
     # device-memory 2x2 EVD
-    for name in ("vjh_dbjac2", "vjh_sbjac2"):
+    for name in ("vjh_dbjac2", "vjh_sbjac2", "vjh_dbjac2i", "vjh_sbjac2i"):
         _decl(L, name, _fint, [_fint, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pu, C.c_void_p])
-    for name in ("vjh_zbjac2", "vjh_cbjac2"):
+    for name in ("vjh_zbjac2", "vjh_cbjac2", "vjh_zbjac2i", "vjh_cbjac2i"):
         _decl(L, name, _fint, [_fint, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pu, C.c_void_p])
     _decl(L, "pjs", P(C.c_uint), [C.c_long, C.c_uint, P(C.c_uint)])
     _decl(L, "pjs_free", None, [P(C.c_uint)])

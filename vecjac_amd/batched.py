@@ -1,4 +1,6 @@
-"""Batched Jacobi SVD of many uneven matrices (two persistent launches per run).
+"""Batched Jacobi SVD of many uneven matrices (one persistent launch per class
+of matrices per run: long columns; short columns on global memory; short
+columns whose G and V stay resident in LDS).
 
 Layout (device memory, one buffer per quantity): matrix b has its m_b x n_b
 G (A on entry, U on exit) at element offset offG[b], column-major with
@@ -32,6 +34,10 @@ def _declare(L):
     L.vjh_bplan_waves.argtypes = [C.c_void_p]
     L.vjh_bplan_evals.restype = C.c_int64
     L.vjh_bplan_evals.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.vjh_bplan_set.restype = C.c_int64
+    L.vjh_bplan_set.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p]
+    L.vjh_bplan_config.restype = C.c_int64
+    L.vjh_bplan_config.argtypes = [C.c_void_p, C.c_void_p]
     L.vjh_bplan_autotune.restype = C.c_int64
     L.vjh_bplan_autotune.argtypes = [C.c_void_p] + [C.c_void_p] * 9 + [C.POINTER(C.c_float), C.c_void_p]
     L._vj_batched_declared = True
@@ -81,10 +87,36 @@ class BatchedSVD:
                              info.data_ptr(), transforms.data_ptr() if transforms is not None else None, s)
         return _lib.check(rc, "vjh_bplan_run")
 
+    def config(self) -> dict:
+        """The plan's choices (vjh_bplan_config): matrices and waves per
+        workgroup of each launch, the split row count, the LDS budget and the
+        LDS bytes per workgroup of the LDS-resident launch."""
+        out = np.zeros(12, dtype=np.int64)
+        _lib.check(_lib.lib().vjh_bplan_config(self.plan, out.ctypes.data), "vjh_bplan_config")
+        names = ("long", "short_global", "short_lds")
+        return {"matrices": dict(zip(names, out[0:3].tolist())), "waves": dict(zip(names, out[3:6].tolist())),
+                "lanes_per_pair": dict(zip(names, out[6:9].tolist())), "split_rows": int(out[9]),
+                "lds_budget": int(out[10]), "lds_bytes_per_workgroup": int(out[11])}
+
+    def set(self, split_rows: int, lds_budget: int, waves=None, lanes=None) -> None:
+        """Fix the split row count, the LDS budget (bytes) and optionally the
+        waves per workgroup of the three launches and the lanes per pair of
+        the two lane-group launches (vjh_bplan_set)."""
+        w = None if waves is None else np.asarray(waves, dtype=np.int32)
+        ln = None if lanes is None else np.asarray(lanes, dtype=np.int32)
+        rc = _lib.lib().vjh_bplan_set(self.plan, int(split_rows), int(lds_budget),
+                                      None if w is None else w.ctypes.data, None if ln is None else ln.ctypes.data)
+        _lib.check(rc, "vjh_bplan_set")
+        if rc < 0:
+            raise ValueError(f"vjh_bplan_set: {_lib.last_error()}")
+        self.waves = _lib.lib().vjh_bplan_waves(self.plan)
+
     def autotune(self, G, V, eS, fS, info, Gi=None, Vi=None, stream=None):
-        """Time the plan at 1, 2, 4 and 8 waves per workgroup on a scratch copy
-        of G (and Gi), keep the fastest (vjh_bplan_autotune).  G is not
-        modified; V, eS, fS and info are.  Returns {waves: ms} of the runs."""
+        """Tune the plan on a scratch copy of G (and Gi): the split row count,
+        the LDS budget and each launch's waves per workgroup
+        (vjh_bplan_autotune); the choices stay in the plan (config()).  G is
+        not modified; V, eS, fS and info are.  Returns {waves: ms} of the
+        runs at the default split."""
         import torch
         L = _lib.lib()
         s = (stream or torch.cuda.current_stream()).cuda_stream

@@ -21,6 +21,12 @@
 // per row of 16 lanes, so that the reductions, the decisions and their
 // scalar work are shared by four pairs instead of two.  The two forms are
 // separate launches because one body holding both exceeds 256 registers.
+//
+// The templates are instantiated once per precision, in svd_batched_d.hip,
+// svd_batched_z.hip, svd_batched_s.hip and svd_batched_c.hip, so that the
+// four compile in parallel.
+#pragma once
+
 #include "svd_common.hpp"
 #include "svd_batched.hpp"
 #include "pjs.hpp"
@@ -441,9 +447,9 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
 // group's pair (the decisions differ between groups, not within one) and
 // the update needs no broadcast.  A permuting update (codes -1, -2) swaps
 // the destination columns instead of the registers.
-template <typename T> __device__ __forceinline__ T group_max(T v, const int LP)
+template <typename T, int LP> __device__ __forceinline__ T group_max(T v)
 {
-  v = row_max(v);
+  v = row_max<T, (LP < 16) ? LP : 16>(v);
   if (LP >= 32) {
     T a = v, b = v;
     swap_<false>(a, b); // a: rows (v0, v0, v2, v2), b: rows (v1, v1, v3, v3)
@@ -456,9 +462,9 @@ template <typename T> __device__ __forceinline__ T group_max(T v, const int LP)
   }
   return v;
 }
-template <typename A> __device__ __forceinline__ DD<A> group_sum(DD<A> v, const int LP)
+template <typename A, int LP> __device__ __forceinline__ DD<A> group_sum(DD<A> v)
 {
-  v = row_sum(v);
+  v = row_sum<A, (LP < 16) ? LP : 16>(v);
   if (LP >= 32) {
     DD<A> a = v, b = v;
     swap_<false>(a.s, b.s);
@@ -667,14 +673,14 @@ template <typename T, bool CPLX, int E> struct GroupShape {
 // per lane, update in registers, store.  Accumulates this wave's per-lane
 // max of the updated G elements, its transformation count and its first
 // error.
-template <typename T, bool CPLX, int E, int W, bool PF>
+template <typename T, bool CPLX, int E, int W, bool PF, int LG>
 __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
-                                           const int lg, const unsigned st, const int wave, const int lane,
+                                           const unsigned st, const int wave, const int lane,
                                            const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr,
                                            unsigned long long &wev)
 {
   typedef typename Acc<T>::A A;
-  const int LP = 1 << lg, P = 64 >> lg; // lanes per group, groups per wave
+  constexpr int lg = LG, LP = 1 << LG, P = 64 >> LG; // lanes per group, groups per wave
   const int npairs = n >> 1;
   if (wave >= npairs)
     return;
@@ -747,16 +753,16 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
     // ---- reduce within the groups
     T mx, my;
     pair_maxima<T, CPLX, E>(xr, yr, xi, yi, nE, mx, my);
-    mx = group_max(mx, LP);
-    my = group_max(my, LP);
+    mx = group_max<T, LP>(mx);
+    my = group_max<T, LP>(my);
     const int kx = max_exp(mx), ky = max_exp(my);
     DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
     pair_sums<T, CPLX, E>(xr, yr, xi, yi, nE, kx, ky, sx, sy, dr, di);
-    sx = group_sum(sx, LP);
-    sy = group_sum(sy, LP);
-    dr = group_sum(dr, LP);
+    sx = group_sum<A, LP>(sx);
+    sy = group_sum<A, LP>(sy);
+    dr = group_sum<A, LP>(dr);
     if constexpr (CPLX)
-      di = group_sum(di, LP);
+      di = group_sum<A, LP>(di);
     // ---- every lane decides its group's pair
     const PairScal<T> Pd = pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
     const PairDec<T> D = pair_decide<T, CPLX>(Pd, S.tol, S.gst);
@@ -831,23 +837,27 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
   }
 }
 
-// Short columns (the launch of the short class, m <= 128 real, m <= 64
-// complex): 16 lanes per pair, four pairs per wave, 4 chunks up to 64 rows
-// and 8 beyond.  (Wider groups lose to the two-pairs-per-wave form of
-// wave_step, whose reductions scatter several sums over the wave.)
-template <typename T, bool CPLX, int E, int W>
+// Short columns (the short classes, m <= 128 real, m <= 64 complex): 2^LG
+// lanes per pair (16, 8 or 4), 64 / 2^LG pairs per wave, with the chunks
+// per lane that 64 rows take, or twice as many beyond 64 rows.  The narrower
+// the group, the more pairs share one instruction stream of reductions and
+// decisions, and the more rows each lane updates.  (Groups wider than a row
+// lose to the two-pairs-per-wave form of wave_step, whose reductions scatter
+// several sums over the wave.)
+template <typename T, bool CPLX, int E, int W, int LG>
 __device__ __forceinline__ void group_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                                const unsigned st, const int wave, const int lane, const Step<T> &S,
                                                T &wG, unsigned long long &wcnt, int &werr, unsigned long long &wev)
 {
+  constexpr int E64 = 64 >> LG; // chunks of 2^LG rows for 64 rows
   if ((E < 2) || CPLX || (m <= 64)) {
-    group_step<T, CPLX, 4, W, GroupShape<T, CPLX, 4>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, 4, st, wave, lane, S, wG, wcnt,
-                                                          werr, wev);
+    group_step<T, CPLX, E64, W, GroupShape<T, CPLX, E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, st, wave, lane, S, wG,
+                                                                  wcnt, werr, wev);
     return;
   }
-  if constexpr ((E >= 2) && !CPLX)
-    group_step<T, CPLX, 8, W, GroupShape<T, CPLX, 8>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, 4, st, wave, lane, S, wG, wcnt,
-                                                          werr, wev);
+  if constexpr ((E >= 2) && !CPLX && (2 * E64 <= 16))
+    group_step<T, CPLX, 2 * E64, W, GroupShape<T, CPLX, 2 * E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, st, wave, lane,
+                                                                          S, wG, wcnt, werr, wev);
 }
 
 // Longer columns: a pair per 64 lanes, two pairs per wave, with the register
@@ -893,9 +903,17 @@ template <typename T, bool CPLX, int E> struct BsvdMinWaves {
       (CPLX ? (E * (int)sizeof(T) * ((sizeof(T) == 8) ? 2 : 1) >= 32) : (E * (int)sizeof(T) >= 128)) ? 1 : 2;
 };
 
-template <typename T, bool CPLX, int E, int W, bool GRP>
+// LDS-resident form (LDS = true, lane-group steps only): the setup scan
+// copies the matrix's G into dynamic LDS, V := I is formed there, every step
+// of every sweep reads and writes those copies, and G (U after the
+// epilogue) and V are copied out once at the end, so a step's latency is
+// that of LDS rather than of a round trip through the caches.
+// LG (lane-group form): 2^LG lanes per pair.
+template <typename T, bool CPLX, int E, int W, bool GRP, bool LDS, int LG>
 __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_bsvd(const BatchArgs<T> a)
 {
+  static_assert(!LDS || GRP, "the LDS-resident form uses the lane-group steps");
+  extern __shared__ double s_dyn[];            // LDS form: G (re, im), then V (re, im)
   __shared__ int s_b;
   __shared__ T s_eS[128], s_fS[128];           // the matrix's (e,f) column norms (n <= 128)
   __shared__ int s_mod[128];                   // step at which each column was last modified
@@ -916,10 +934,16 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       break;
     const int b = a.order[bi];
     const int m = (int)a.m[b], n = (int)a.n[b];
-    T *const Gr = a.Gr + a.offG[b];
-    T *const Gi = CPLX ? a.Gi + a.offG[b] : nullptr;
-    T *const Vr = a.Vr + a.offV[b];
-    T *const Vi = CPLX ? a.Vi + a.offV[b] : nullptr;
+    T *const gGr = a.Gr + a.offG[b];
+    T *const gGi = CPLX ? a.Gi + a.offG[b] : nullptr;
+    T *const gVr = a.Vr + a.offV[b];
+    T *const gVi = CPLX ? a.Vi + a.offV[b] : nullptr;
+    T *const lG = (T *)s_dyn;
+    T *const lV = lG + (CPLX ? 2 : 1) * m * n;
+    T *const Gr = LDS ? lG : gGr;
+    T *const Gi = CPLX ? (LDS ? lG + m * n : gGi) : nullptr;
+    T *const Vr = LDS ? lV : gVr;
+    T *const Vi = CPLX ? (LDS ? lV + n * n : gVi) : nullptr;
     T *const eS = a.eS + a.offS[b];
     T *const fS = a.fS + a.offS[b];
     const unsigned stp = (unsigned)n; // steps per sweep of the modulus ordering
@@ -930,9 +954,16 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     // ---- setup (dvjsvd.c:78-184): max |G|, V := I, eS/fS, initial scale
     T mxv = (T)0;
     for (long long t = threadIdx.x; t < (long long)m * n; t += 64 * W) {
-      mxv = xmax(xmin(vabs(Gr[t]), inf_<T>()), mxv);
-      if constexpr (CPLX)
-        mxv = xmax(xmin(vabs(Gi[t]), inf_<T>()), mxv);
+      const T g = gGr[t];
+      mxv = xmax(xmin(vabs(g), inf_<T>()), mxv);
+      if constexpr (LDS)
+        Gr[t] = g;
+      if constexpr (CPLX) {
+        const T gi = gGi[t];
+        mxv = xmax(xmin(vabs(gi), inf_<T>()), mxv);
+        if constexpr (LDS)
+          Gi[t] = gi;
+      }
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1)
@@ -995,7 +1026,8 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           unsigned long long wcnt = 0;
           int werr = 0;
           if constexpr (GRP)
-            group_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr, wev);
+            group_dispatch<T, CPLX, E, W, LG>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr,
+                                              wev);
           else
             wave_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr, s_mod,
                                          gstep, wev);
@@ -1064,6 +1096,22 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           s_eS[j] -= (T)sT;
       }
     }
+    if constexpr (LDS) {
+      // G (U after the epilogue, else the working G) and V out of LDS
+      __syncthreads();
+      if (norms) {
+        for (long long t = threadIdx.x; t < (long long)m * n; t += 64 * W) {
+          gGr[t] = Gr[t];
+          if constexpr (CPLX)
+            gGi[t] = Gi[t];
+        }
+        for (int t = threadIdx.x; t < n * n; t += 64 * W) {
+          gVr[t] = Vr[t];
+          if constexpr (CPLX)
+            gVi[t] = Vi[t];
+        }
+      }
+    }
     if (lane == 0 && wev)
       atomicAdd(&s_ev, wev);
     __syncthreads();
@@ -1082,50 +1130,86 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
   }
 }
 
-template <typename T, bool CPLX, int E, int W, bool GRP>
+template <typename T, bool CPLX, int E, int W, bool GRP, bool LDS, int LG>
 static int launch_bsvd(const BatchArgs<T> &a, int grid, hipStream_t s)
 {
-  hipLaunchKernelGGL((k_bsvd<T, CPLX, E, W, GRP>), dim3(grid), dim3(64 * W), 0, s, a);
+  const size_t lds = LDS ? (size_t)a.lds_bytes : 0;
+  if (lds > 64 * 1024)
+    VJ_HIP(hipFuncSetAttribute((const void *)k_bsvd<T, CPLX, E, W, GRP, LDS, LG>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_bsvd<T, CPLX, E, W, GRP, LDS, LG>), dim3(grid), dim3(64 * W), lds, s, a);
   return check_launch();
 }
 
-template <typename T, bool CPLX, int E, int W, bool GRP>
-static int occupancy(int *blocks)
+template <typename T, bool CPLX, int E, int W, bool GRP, bool LDS, int LG>
+static int occupancy(int *blocks, size_t lds)
 {
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_bsvd<T, CPLX, E, W, GRP>, 64 * W, 0);
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_bsvd<T, CPLX, E, W, GRP, LDS, LG>, 64 * W, lds);
   return (e == hipSuccess) ? 0 : set_error(VJ_EHIP - (int)e, "occupancy query failed");
 }
 
-template <typename T, bool CPLX, int W, bool GRP>
+template <typename T, bool CPLX, int W, bool GRP, bool LDS, int LG>
 static int dispatch_E(const BatchArgs<T> &a, int max_m, int grid_cap, hipStream_t s)
 {
   int cus = 0, blocks = 1, dev = 0;
   VJ_HIP(hipGetDevice(&dev));
   VJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const size_t lds = LDS ? (size_t)a.lds_bytes : 0;
 #define VJ_BS_CASE(EE)                                                         \
   {                                                                            \
-    if (int rc = occupancy<T, CPLX, EE, W, GRP>(&blocks))                      \
+    if (int rc = occupancy<T, CPLX, EE, W, GRP, LDS, LG>(&blocks, lds))        \
       return rc;                                                               \
     long long g = (long long)cus * (blocks > 0 ? blocks : 1);                  \
     if (grid_cap > 0 && g > grid_cap) g = grid_cap;                            \
     if (g > a.nb) g = a.nb;                                                    \
-    return launch_bsvd<T, CPLX, EE, W, GRP>(a, (int)g, s);                     \
+    return launch_bsvd<T, CPLX, EE, W, GRP, LDS, LG>(a, (int)g, s);            \
   }
   if (max_m <= 64) VJ_BS_CASE(1)
-  if (max_m <= 128) VJ_BS_CASE(2)
+  if constexpr (!GRP || group_shape_ok(CPLX, 128, LG, W))
+    if (max_m <= 128) VJ_BS_CASE(2)
   if constexpr (!GRP) {
     if (max_m <= 256) VJ_BS_CASE(4)
     if (max_m <= 512) VJ_BS_CASE(8)
     if (max_m <= 1024) VJ_BS_CASE(16)
   }
 #undef VJ_BS_CASE
-  return set_error(-1, "batched SVD: m = %d exceeds %d", max_m, GRP ? 128 : 1024);
+  return set_error(-1, "batched SVD: m = %d exceeds %d", max_m, GRP ? (int)group_max_rows(CPLX, LG) : 1024);
+}
+
+template <typename T, bool CPLX, int W, bool LDS, int LG>
+static int dispatch_LG(const BatchArgs<T> &a, int max_m, int grid_cap, hipStream_t s)
+{
+  if constexpr (group_shape_ok(CPLX, 64, LG, W))
+    return dispatch_E<T, CPLX, W, true, LDS, LG>(a, max_m, grid_cap, s);
+  return set_error(-1, "batched SVD: no lane-group form with %d lanes per pair and %d waves", 1 << LG, W);
+}
+
+template <typename T, bool CPLX, int W, bool LDS>
+static int dispatch_grp(const BatchArgs<T> &a, int max_m, int grid_cap, hipStream_t s)
+{
+  if (max_m > group_max_rows(CPLX, a.group_lg))
+    return set_error(-1, "batched SVD: m = %d exceeds the %d rows of the lane-group form with %d lanes per pair",
+                     max_m, group_max_rows(CPLX, a.group_lg), 1 << a.group_lg);
+  switch (a.group_lg) {
+  case 4: return dispatch_LG<T, CPLX, W, LDS, 4>(a, max_m, grid_cap, s);
+  case 3: return dispatch_LG<T, CPLX, W, LDS, 3>(a, max_m, grid_cap, s);
+  case 2: return dispatch_LG<T, CPLX, W, LDS, 2>(a, max_m, grid_cap, s);
+  default: return set_error(-1, "batched SVD: %d lanes per pair (16, 8 or 4)", 1 << a.group_lg);
+  }
 }
 
 template <typename T, bool CPLX, int W>
 static int dispatch_G(const BatchArgs<T> &a, int max_m, int grid_cap, bool grp, hipStream_t s)
 {
-  return grp ? dispatch_E<T, CPLX, W, true>(a, max_m, grid_cap, s) : dispatch_E<T, CPLX, W, false>(a, max_m, grid_cap, s);
+  if (!grp)
+    return dispatch_E<T, CPLX, W, false, false, 4>(a, max_m, grid_cap, s);
+  if (a.lds_bytes > 0) {
+    if (a.lds_bytes > LDS_FORM_MAX_BYTES)
+      return set_error(-1, "batched SVD: %d bytes of LDS per matrix exceed %d", a.lds_bytes, LDS_FORM_MAX_BYTES);
+    return dispatch_grp<T, CPLX, W, true>(a, max_m, grid_cap, s);
+  }
+  return dispatch_grp<T, CPLX, W, false>(a, max_m, grid_cap, s);
 }
 
 template <typename T, bool CPLX>
@@ -1141,10 +1225,5 @@ int batched_svd(const BatchArgs<T> &a, int waves, int max_m, int grid_cap, bool 
   default: return set_error(-1, "batched SVD: waves per workgroup must be 1, 2, 4 or 8 (got %d)", waves);
   }
 }
-
-template int batched_svd<double, false>(const BatchArgs<double> &, int, int, int, bool, hipStream_t);
-template int batched_svd<double, true>(const BatchArgs<double> &, int, int, int, bool, hipStream_t);
-template int batched_svd<float, false>(const BatchArgs<float> &, int, int, int, bool, hipStream_t);
-template int batched_svd<float, true>(const BatchArgs<float> &, int, int, int, bool, hipStream_t);
 
 } // namespace vj

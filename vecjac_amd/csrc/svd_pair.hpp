@@ -135,19 +135,27 @@ template <bool H32, typename T> __device__ __forceinline__ DD<T> dd_swapadd(DD<T
   return dd_add(a, b);
 }
 
-// row totals (every lane of a row of 16) of a max / a double-double sum
-template <typename T> __device__ __forceinline__ T row_max(T v)
+// totals (every lane of an aligned group of LP = 16, 8 or 4 lanes of a row)
+// of a max / a double-double sum: the mirror pairs the halves of a row, the
+// half mirror the halves of an 8-lane group, the quad exchanges the rest
+template <typename T, int LP = 16> __device__ __forceinline__ T row_max(T v)
 {
-  v = xmax(v, dpp_<DPP_MIRROR>(v));
-  v = xmax(v, dpp_<DPP_HMIRROR>(v));
+  static_assert(LP == 16 || LP == 8 || LP == 4, "row groups of 16, 8 or 4 lanes");
+  if constexpr (LP >= 16)
+    v = xmax(v, dpp_<DPP_MIRROR>(v));
+  if constexpr (LP >= 8)
+    v = xmax(v, dpp_<DPP_HMIRROR>(v));
   v = xmax(v, dpp_<DPP_XOR2>(v));
   v = xmax(v, dpp_<DPP_XOR1>(v));
   return v;
 }
-template <typename T> __device__ __forceinline__ DD<T> row_sum(DD<T> v)
+template <typename T, int LP = 16> __device__ __forceinline__ DD<T> row_sum(DD<T> v)
 {
-  v = dd_add(v, dd_dpp<DPP_MIRROR>(v));
-  v = dd_add(v, dd_dpp<DPP_HMIRROR>(v));
+  static_assert(LP == 16 || LP == 8 || LP == 4, "row groups of 16, 8 or 4 lanes");
+  if constexpr (LP >= 16)
+    v = dd_add(v, dd_dpp<DPP_MIRROR>(v));
+  if constexpr (LP >= 8)
+    v = dd_add(v, dd_dpp<DPP_HMIRROR>(v));
   v = dd_add(v, dd_dpp<DPP_XOR2>(v));
   v = dd_add(v, dd_dpp<DPP_XOR1>(v));
   return v;

@@ -20,5 +20,10 @@ template <typename T>
 int launch_rbjac2(long long n, const T *a11, const T *a22, const T *a21, T *c, T *at, T *l1, T *l2, unsigned *p, hipStream_t s);
 template <typename T>
 int launch_cbjac2(long long n, const T *a11, const T *a22, const T *a21r, const T *a21i, T *c, T *cat, T *sat, T *l1, T *l2, unsigned *p, hipStream_t s);
+// internal forms: blocks gated by the low bits of p (in and out)
+template <typename T>
+int launch_rbjac2i(long long n, const T *a11, const T *a22, const T *a21, T *c, T *at, T *l1, T *l2, unsigned *p, hipStream_t s);
+template <typename T>
+int launch_cbjac2i(long long n, const T *a11, const T *a22, const T *a21r, const T *a21i, T *c, T *cat, T *sat, T *l1, T *l2, unsigned *p, hipStream_t s);
 
 } // namespace vj

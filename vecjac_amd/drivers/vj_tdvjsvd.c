@@ -64,6 +64,7 @@ int main(int argc, char *argv[])
   const long double t0 = vj_now();
   const vj_fint o = dvjsvd_(&m, &n, G, &ldG, V, &ldV, eS, fS, js, &stp, &swp, work, iwork);
   const long double t1 = vj_now();
+  vjh_set_jtrace(0); /* the switch is process-wide: off again once this call is traced */
   (void)fprintf(stdout, "\"%s\",%1ld,%4llu,%4llu,%15.9Lf,%3lld\n", bn, j, (unsigned long long)m, (unsigned long long)n,
                 t1 - t0, (long long)o);
   (void)fflush(stdout);

@@ -67,6 +67,7 @@ int main(int argc, char *argv[])
   t = vj_now();
   const vj_fint o = zvjsvd_(&m, &n, Gr, &ld, Gi, &ld, Vr, &ldV, Vi, &ldV, eS, fS, js, &stp, &swp, work, iwork);
   const long double tj = vj_now() - t;
+  vjh_set_jtrace(0); /* the switch is process-wide: off again once this call is traced */
   free(iwork);
   if (vj_write_sigma(bn, (j == 2) ? "S2" : "S4", n, eS, fS))
     return 10;
