@@ -159,11 +159,13 @@ def cpu_bjac2(kind: str, ins_np):
 
 # --------------------------------------------------------------------------- batched SVD
 def _bsvd_alg_bytes(m, n, evals, transforms, w):
-    """Algorithmic HBM bytes of one batched run: every pair evaluation reads
-    the pair's two G columns (w 2m) and a transformed pair writes them and
-    reads/writes its two V columns (w (2m + 4n)).  evals counts the pair
-    evaluations the run made (a pair whose columns did not change since its
-    previous evaluation is not evaluated again)."""
+    """Per-step column bytes of one batched run, as if every pair moved
+    through memory: every pair evaluation reads the pair's two G columns
+    (w 2m) and a transformed pair writes them and reads/writes its two V
+    columns (w (2m + 4n)).  evals counts the pair evaluations the run made (a
+    pair whose columns did not change since its previous evaluation is not
+    evaluated again).  (The LDS-resident form moves these bytes between LDS
+    and registers.)"""
     m = m.astype(np.float64)
     n = n.astype(np.float64)
     return float(np.sum(evals.astype(np.float64) * w * 2 * m + transforms.astype(np.float64) * w * (2 * m + 4 * n)))
@@ -193,7 +195,13 @@ def bench_bsvd(args, kind: str = "d"):
         tr = torch.empty(nb, dtype=torch.int64, device="cuda")
         stream = torch.cuda.current_stream()
         tune = None
-        if args.waves == 0 and not args.no_tune:
+        if args.order:
+            plan.order(args.order)
+        if args.plan:
+            sp, bud, wv, ln = args.plan.split(":")
+            plan.set(int(sp), int(bud), [int(x) for x in wv.split(",")], [16] + [int(x) for x in ln.split(",")])
+            tune = {"fixed": args.plan}
+        elif args.waves == 0 and not args.no_tune:
             # plan-time autotuner (split, LDS budget, waves per workgroup),
             # outside the timed region
             t_tune = time.perf_counter()
@@ -215,7 +223,7 @@ def bench_bsvd(args, kind: str = "d"):
         kms = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)]
         res = dict(plan=plan, wall=wall, kernel_ms=float(np.mean(kms)), info=info.cpu().numpy(),
                    tr=tr.cpu().numpy(), G=G, eS=eS.cpu().numpy(), fS=fS.cpu().numpy(), waves=plan.waves, tune=tune,
-                   evals=plan.evals(), plan_config=plan.config())
+                   evals=plan.evals(), xfer=plan.traffic(), plan_config=plan.config(), order=plan.order())
         return res
 
     w = 8 if kind == "d" else 4
@@ -223,7 +231,11 @@ def bench_bsvd(args, kind: str = "d"):
     info, tr = r["info"], r["tr"]
     if np.any(info < 0):
         raise RuntimeError(f"batched SVD failed on {int(np.sum(info < 0))} matrices")
-    alg = _bsvd_alg_bytes(m, npad, r["evals"], tr, w)
+    # algorithmic bytes: the G and V elements the run's design moves to and
+    # from memory (every column load and store of the steps or block rounds,
+    # the set-up scan, V := I, the scaling and the epilogue; G in and U, V out
+    # for a matrix held in LDS), counted by the kernel (vjh_bplan_traffic)
+    alg = float(np.sum(r["xfer"].astype(np.float64))) * w
     k_s = r["kernel_ms"] / 1e3
     achieved = alg / k_s / 1e9
     value = nb * args.steps / r["wall"]
@@ -236,7 +248,7 @@ def bench_bsvd(args, kind: str = "d"):
                                "n ~ U{8..128} (odd n padded with a zero column), m ~ U{n..4n}, N(0,1) entries, "
                                "one persistent launch per class (long columns; short on global memory; short "
                                "resident in LDS)",
-                   "batch": nb, "plan": r["plan_config"], "autotune": r["tune"],
+                   "batch": nb, "ordering": f"pjs id {r['order']}", "plan": r["plan_config"], "autotune": r["tune"],
                    "sweeps_mean": float(np.mean(info)), "sweeps_max": int(np.max(info)),
                    "pair_evaluations": int(np.sum(r["evals"])),
                    "pair_slots": int(np.sum((np.minimum(info, 998) + 1) * npad * (npad // 2)))},
@@ -278,7 +290,7 @@ def _bsvd_sigma_check(m, npad, offG, offS, G64, r, kind, nsamp=6):
     for b in idx:
         mm, nn = int(m[b]), int(npad[b])
         A = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm).astype(np.float64 if kind == "d" else np.float32)
-        js, stp = S.pjs(4, nn)
+        js, stp = S.pjs(6, nn)
         Go = vjo.aligned((nn, mm), A.dtype)
         Go[...] = A
         sg = np.sort(S.sigma(r["eS"][offS[b]: offS[b] + nn], r["fS"][offS[b]: offS[b] + nn]))
@@ -305,7 +317,7 @@ def _bsvd_plain_sample(m, npad, offG, G64, kind, nsamp):
     for b in idx:
         mm, nn = int(m[b]), int(npad[b])
         A = G64[offG[b]: offG[b] + mm * nn].reshape(nn, mm).astype(dt_np)
-        js, stp = S.pjs(4, nn)
+        js, stp = S.pjs(6, nn)
         mats.append((torch.from_numpy(A).cuda(), torch.empty((nn, nn), dtype=dt, device="cuda"),
                      torch.empty(nn, dtype=dt, device="cuda"), torch.empty(nn, dtype=dt, device="cuda"),
                      torch.from_numpy(js.reshape(-1).view(np.int32)).cuda(), stp))
@@ -341,7 +353,7 @@ def _bsvd_cpu(m_real, n_real, m, offG, npad, G64, kind):
 
     def table(nn):
         if nn not in tables:
-            js, stp = S.pjs(4, nn)
+            js, stp = S.pjs(6, nn)
             jsa = vjo.aligned(js.shape, np.uint32)
             jsa[...] = js
             tables[nn] = (jsa, stp)
@@ -728,6 +740,9 @@ def main() -> int:
     ap.add_argument("--lg", type=int, default=26, help="log2 batch size of the 2x2 workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="skip the batched plan's autotuner")
+    ap.add_argument("--plan", default="", help="fixed batched plan instead of the autotuner, as "
+                    "split_rows:lds_budget:w0,w1,w2:l1,l2 (waves per launch, lanes per pair of the short launches)")
+    ap.add_argument("--order", type=int, default=0, help="pjs id of the batched ordering (4 or 6; 0: the plan's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "

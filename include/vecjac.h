@@ -108,7 +108,11 @@ vj_fint cvjsvd_(const vj_fnat *m, const vj_fnat *n, float *Gr, const vj_fnat *ld
 
 /* Parallel Jacobi strategy table; replaces pjs() (pjs.h:18, pjs.c:7).
  * id 2 (ME): n-1 steps; id 4 (MM): n steps; n even.  Returns a malloc'd
- * stp x n table (free with free() or pjs_free()), NULL on bad arguments. */
+ * stp x n table (free with free() or pjs_free()), NULL on bad arguments.
+ * Extension (no reference counterpart): id 6, the block modulus ordering of
+ * the batched SVD, n steps (blocks {2i, 2i+1}; round s meets block I with
+ * block s - I mod n/2 in two steps), so that the plain drivers can run the
+ * batched kernel's sequence of pairs. */
 unsigned *pjs(long id, unsigned n, unsigned *stp);
 
 /* Interleaved -> split complex layout; replaces zsplit_ (zsplit.h:6,
@@ -253,6 +257,18 @@ vj_fint vjh_bplan_set(void *plan, int split_rows, vj_fint lds_budget, const int 
  * evaluated again, so this can be below sweeps x steps x n/2; it is what the
  * run read.  Synchronises `stream`. */
 vj_fint vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream);
+/* Elements of G and V (both planes of complex data) that each matrix's
+ * solve in the plan's last run read from or wrote to device memory
+ * (uint64_t elems[nb]): the column loads and stores of every step, the
+ * set-up scan, V := I, the initial scaling and the epilogue; for a matrix
+ * held in LDS, G read once and U and V written once.  Synchronises `stream`. */
+vj_fint vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream);
+/* Ordering of the pairs in the plan's runs: 6 (default), the block modulus
+ * ordering of pjs(6, n), whose long-column launch keeps the four columns of
+ * a block pair in registers for the two steps of a round; or 4, the modulus
+ * ordering of pjs(4, n).  A run equals the plain driver run with the same
+ * table.  id 0 returns the current ordering; another id returns -2. */
+vj_fint vjh_bplan_order(void *plan, int id);
 /* Autotuner of a plan, on copies of the input G (Gi) made into the scratch
  * buffer Gw (Gwi), each launch timed on its own with HIP events on `stream`:
  * (1) for the split row count and half of it, each launch at 1, 2, 4 and 8

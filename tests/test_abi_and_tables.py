@@ -71,8 +71,38 @@ def test_pjs_mm(n):
     assert int(cover[iu].sum()) == n * n // 2
 
 
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 10, 16, 30, 64, 126, 128])
+def test_pjs_block_modulus(n):
+    """id 6 against its definition: round s meets block I = {2I, 2I+1} with
+    block J = s - I mod n/2, crossing (2I, 2J), (2I+1, 2J+1) at step 2s and
+    (2I, 2J+1), (2I+1, 2J) at step 2s+1, a block met by itself pairing its
+    own columns at both steps."""
+    from vecjac_amd import svd as S
+    t, stp = S.pjs(6, n)
+    assert stp == n and t.shape == (n, n)
+    nb = n // 2
+    cover = np.zeros((n, n), dtype=int)
+    for st, r in enumerate(t):
+        assert np.array_equal(np.sort(r), np.arange(n))
+        p, q = r[0::2], r[1::2]
+        assert np.all(p < q)
+        cover[p, q] += 1
+        s, h = divmod(st, 2)
+        want = set()
+        for I in range(nb):
+            J = (s - I) % nb
+            if I == J:
+                want.add((2 * I, 2 * I + 1))
+            elif I < J:
+                want |= {(2 * I, 2 * J + h), (2 * I + 1, 2 * J + 1 - h)}
+        assert set(zip(p.tolist(), q.tolist())) == want
+    iu = np.triu_indices(n, 1)
+    inblock = (iu[1] == iu[0] + 1) & (iu[0] % 2 == 0)
+    assert np.all(cover[iu][inblock] == 2) and np.all(cover[iu][~inblock] == 1)
+
+
 def test_pjs_refusals():
     from vecjac_amd import svd as S
-    for args in ((2, 7), (4, 1), (3, 8), (2, 0)):
+    for args in ((2, 7), (4, 1), (3, 8), (2, 0), (6, 5), (5, 8)):
         with pytest.raises(ValueError):
             S.pjs(*args)

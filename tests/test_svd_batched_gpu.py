@@ -26,10 +26,12 @@ def _batch(nb, seed, nmax=48, dtype=np.float64):
     return m, n, npad, offG, offV, offS, tot, G
 
 
-def _run(kind, m, npad, G, tot, waves=0, Gi=None, split=None, budget=None, wv=None, lanes=None):
+def _run(kind, m, npad, G, tot, waves=0, Gi=None, split=None, budget=None, wv=None, lanes=None, order=None):
     import torch
     dt = torch.float64 if kind in "dz" else torch.float32
     plan = BT.BatchedSVD(m, npad, kind=kind, waves=waves)
+    if order is not None:
+        assert plan.order(order) == order
     if budget is not None:
         plan.set(split if split is not None else (64 if kind in "zc" else 128), budget, wv, lanes)
     Gd = torch.from_numpy(G.copy()).cuda()
@@ -40,12 +42,13 @@ def _run(kind, m, npad, G, tot, waves=0, Gi=None, split=None, budget=None, wv=No
     fS = torch.zeros(tot[2], dtype=dt, device="cuda")
     info = torch.zeros(len(m), dtype=torch.int64, device="cuda")
     tr = torch.zeros(len(m), dtype=torch.int64, device="cuda")
-    print(f"batched {kind} nb={len(m)} waves={plan.waves}", flush=True)
+    print(f"batched {kind} nb={len(m)} waves={plan.waves} order={plan.order()}", flush=True)
     rc = plan.run(Gd, V, eS, fS, info, tr, Gi=Gid, Vi=Vi)
     torch.cuda.synchronize()
-    plan.close()
     out = dict(rc=rc, G=Gd.cpu().numpy(), V=V.cpu().numpy(), eS=eS.cpu().numpy(), fS=fS.cpu().numpy(),
-               info=info.cpu().numpy(), tr=tr.cpu().numpy())
+               info=info.cpu().numpy(), tr=tr.cpu().numpy(), evals=plan.evals().astype(np.int64),
+               xfer=plan.traffic().astype(np.int64))
+    plan.close()
     if Gi is not None:
         out["Gi"] = Gid.cpu().numpy()
         out["Vi"] = Vi.cpu().numpy()
@@ -61,7 +64,7 @@ def test_batched_vs_oracle_f64(waves):
     for b in range(nb):
         mm, nn = int(m[b]), int(npad[b])
         A = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
-        js, stp = S.pjs(4, nn)
+        js, stp = S.pjs(6, nn)
         sg = S.sigma(r["eS"][offS[b]: offS[b] + nn], r["fS"][offS[b]: offS[b] + nn])
         for mode in (1, 0):  # the GPU's norm and dot; the reference's own
             Go = vjo.aligned((nn, mm), np.float64)
@@ -89,7 +92,7 @@ def test_batched_equals_plain_bitwise():
     for b in range(nb):
         mm, nn = int(m[b]), int(npad[b])
         A = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
-        js, stp = S.pjs(4, nn)
+        js, stp = S.pjs(6, nn)
         Gd = torch.from_numpy(A.copy()).cuda()
         Vd = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
         eS = torch.empty(nn, dtype=torch.float64, device="cuda")
@@ -139,22 +142,25 @@ def test_batched_nonfinite_and_zero_members():
     assert r["info"][0] >= 1 and r["info"][2] >= 1
 
 
-def test_batched_wide_rows_vs_plain_and_oracle():
+@pytest.mark.parametrize("order", [6, 4])
+def test_batched_wide_rows_vs_plain_and_oracle(order):
     """Widths 64..128 and up to 512 rows: the register footprints of 4 and 8
-    row chunks per lane (two pairs per wave, with and without prefetch) against
-    the plain driver bit for bit, and two members against the oracle."""
+    row chunks per lane (block rounds of four columns with and without
+    prefetch under ordering 6; two pairs per wave under ordering 4) against
+    the plain driver run with the same table bit for bit, and two members
+    against the oracle."""
     import torch
     g = W.rng(321)
     n = np.array([128, 96, 64, 128, 80, 112], dtype=np.int64)
     m = np.array([512, 300, 256, 130, 500, 448], dtype=np.int64)
     _, npad, offG, offV, offS, tot = BT.pack_shapes(m, n)
     G = BT.fill_gaussian(m, npad, n, offG, 322)
-    r = _run("d", m, npad, G, tot)
+    r = _run("d", m, npad, G, tot, order=order)
     assert r["rc"] == 0 and np.all(r["info"] > 0)
     for b in range(len(m)):
         mm, nn = int(m[b]), int(npad[b])
         A = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
-        js, stp = S.pjs(4, nn)
+        js, stp = S.pjs(order, nn)
         Gd = torch.from_numpy(A.copy()).cuda()
         Vd = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
         eS = torch.empty(nn, dtype=torch.float64, device="cuda")
@@ -169,7 +175,7 @@ def test_batched_wide_rows_vs_plain_and_oracle():
         assert np.array_equal(fS.cpu().numpy(), r["fS"][offS[b]: offS[b] + nn])
     for b in (1, 3):
         mm, nn = int(m[b]), int(npad[b])
-        js, stp = S.pjs(4, nn)
+        js, stp = S.pjs(order, nn)
         Go = vjo.aligned((nn, mm), np.float64)
         Go[...] = G[offG[b]: offG[b] + mm * nn].reshape(nn, mm)
         with vjo.norm_mode(1):
@@ -246,22 +252,23 @@ def test_batched_autotune_keeps_results():
     plan.close()
 
 
-@pytest.mark.parametrize("kind", ["d", "z"])
-def test_short_form_vs_plain_bitwise(kind):
+@pytest.mark.parametrize("kind,order", [("d", 6), ("d", 4), ("z", 6), ("z", 4)])
+def test_short_form_vs_plain_bitwise(kind, order):
     """Members on both sides of the lane-group bounds (4 chunks of 16 lanes up
     to 64 rows, 8 chunks up to 128; complex up to 64 rows) and just past them
-    in the wave form, against the plain driver bit for bit."""
+    in the wave form (block rounds for real data under ordering 6), against
+    the plain driver run with the same table bit for bit."""
     import torch
     m = np.array([8, 16, 40, 63, 64, 65, 96, 127, 128, 129, 200], dtype=np.int64)
     n = np.array([8, 10, 40, 30, 64, 34, 48, 64, 128, 66, 100], dtype=np.int64)
     _, npad, offG, offV, offS, tot = BT.pack_shapes(m, n)
     G = BT.fill_gaussian(m, npad, n, offG, 431)
     Gi = BT.fill_gaussian(m, npad, n, offG, 432) if kind == "z" else None
-    r = _run(kind, m, npad, G, tot, Gi=Gi)
+    r = _run(kind, m, npad, G, tot, Gi=Gi, order=order)
     assert r["rc"] == 0 and np.all(r["info"] > 0)
     for b in range(len(m)):
         mm, nn = int(m[b]), int(npad[b])
-        js, stp = S.pjs(4, nn)
+        js, stp = S.pjs(order, nn)
         Gd = torch.from_numpy(G[offG[b]: offG[b] + mm * nn].reshape(nn, mm).copy()).cuda()
         Vd = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
         Gid = Vid = None
@@ -344,3 +351,26 @@ def test_lds_resident_form_bitwise(kind):
     for key, r in runs.items():
         for f in ("G", "V", "eS", "fS", "info", "tr") + (("Gi", "Vi") if kind in "zc" else ()):
             assert np.array_equal(r[f], ref[f], equal_nan=True), (key, f)
+
+
+def test_block_rounds_halve_column_traffic():
+    """vjh_bplan_traffic: the long-column members move about half the G and
+    V elements per pair evaluation in block rounds (ordering 6: four columns
+    loaded and stored once for four evaluations) that they move in single
+    steps (ordering 4: two columns per evaluation); both counts cover at
+    least the set-up scan, V := I and one sweep of loads."""
+    m = np.array([160, 256, 300, 512, 400, 200], dtype=np.int64)
+    n = np.array([32, 64, 96, 128, 80, 40], dtype=np.int64)
+    _, npad, offG, offV, offS, tot = BT.pack_shapes(m, n)
+    G = BT.fill_gaussian(m, npad, n, offG, 515)
+    per = {}
+    for order in (6, 4):
+        r = _run("d", m, npad, G, tot, order=order)
+        assert r["rc"] == 0 and np.all(r["info"] > 0)
+        nn = npad.astype(np.int64)
+        floor = m * nn + nn * nn + (nn // 2) * nn * (m + nn)
+        assert np.all(r["xfer"] >= floor), (order, r["xfer"], floor)
+        per[order] = r["xfer"] / r["evals"]
+    ratio = per[6] / per[4]
+    print("traffic per evaluation, ordering 6 / ordering 4:", np.round(ratio, 3))
+    assert np.all(ratio < 0.75), ratio

@@ -34,6 +34,10 @@ def _declare(L):
     L.vjh_bplan_waves.argtypes = [C.c_void_p]
     L.vjh_bplan_evals.restype = C.c_int64
     L.vjh_bplan_evals.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.vjh_bplan_traffic.restype = C.c_int64
+    L.vjh_bplan_traffic.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.vjh_bplan_order.restype = C.c_int64
+    L.vjh_bplan_order.argtypes = [C.c_void_p, C.c_int]
     L.vjh_bplan_set.restype = C.c_int64
     L.vjh_bplan_set.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p]
     L.vjh_bplan_config.restype = C.c_int64
@@ -140,6 +144,23 @@ class BatchedSVD:
         s = (stream or torch.cuda.current_stream()).cuda_stream
         _lib.check(_lib.lib().vjh_bplan_evals(self.plan, out.ctypes.data, s), "vjh_bplan_evals")
         return out
+
+    def traffic(self, stream=None) -> np.ndarray:
+        """Elements of G and V each matrix's solve moved to and from device
+        memory in the last run (vjh_bplan_traffic)."""
+        import torch
+        out = np.zeros(len(self.m), dtype=np.uint64)
+        s = (stream or torch.cuda.current_stream()).cuda_stream
+        _lib.check(_lib.lib().vjh_bplan_traffic(self.plan, out.ctypes.data, s), "vjh_bplan_traffic")
+        return out
+
+    def order(self, id_: int = 0) -> int:
+        """Set the ordering of the pairs (pjs id 6, the default, or 4) and
+        return it; 0 only returns it (vjh_bplan_order)."""
+        rc = _lib.lib().vjh_bplan_order(self.plan, int(id_))
+        if rc < 0:
+            raise ValueError(f"vjh_bplan_order: {_lib.last_error()}")
+        return int(rc)
 
     def close(self):
         if getattr(self, "plan", None):

@@ -2,7 +2,8 @@
 // shape-dependent metadata on the device (descriptors and the cost order),
 // so that a run is a single kernel launch on resident data and can be
 // captured in a HIP graph.  The kernel computes the pairs of each step
-// itself: the modulus ordering, the table pjs(4, n).
+// itself: the block modulus ordering, the table pjs(6, n), by default, or
+// the modulus ordering, pjs(4, n).
 #include "svd_batched.hpp"
 #include "svd_common.hpp"
 #include "vecjac.h"
@@ -39,12 +40,14 @@ struct BPlan {
   int lds_bytes = 0;       // dynamic LDS per workgroup of the LDS launch (max over its class)
   long long totG = 0;      // elements of the packed G (max over b of offG + m n)
   unsigned swp = 999u;
+  int ord = 6;             // ordering of the pairs (pjs id)
   int dev = 0;
   std::vector<long long> hm, hn; // the shapes, for re-partitioning
   long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each)
   int *d_order = nullptr;
   unsigned *d_queue = nullptr; // work counters of the launches
   unsigned long long *d_evals = nullptr; // pair evaluations per matrix (last run)
+  unsigned long long *d_xfer = nullptr;  // elements moved per matrix (last run)
 };
 
 __global__ void k_zero_u32(unsigned *p)
@@ -140,7 +143,9 @@ int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, v
   a.info = (long long *)info;
   a.transforms = (unsigned long long *)transforms;
   a.evals = P->d_evals;
+  a.xfer = P->d_xfer;
   a.swp = P->swp;
+  a.ord = P->ord;
   // a fill kernel, not hipMemsetAsync, so that the run can be graph-captured
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, P->d_queue);
   // the long columns first (the costly matrices start first), then the
@@ -325,6 +330,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_order, std::max<size_t>(1, (size_t)nb) * sizeof(int));
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, NCLASS * sizeof(unsigned));
   if (e == hipSuccess) e = hipMalloc((void **)&P->d_evals, std::max<size_t>(1, (size_t)nb) * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void **)&P->d_xfer, std::max<size_t>(1, (size_t)nb) * sizeof(unsigned long long));
   if (e == hipSuccess && !meta.empty()) e = hipMemcpy(P->d_meta, meta.data(), meta.size() * sizeof(long long), hipMemcpyHostToDevice);
   int rc = (e == hipSuccess) ? partition(P) : set_error(VJ_EHIP - (int)e, "plan allocation failed: %s", hipGetErrorString(e));
   if (rc) {
@@ -333,6 +339,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
     (void)hipFree(P->d_order);
     (void)hipFree(P->d_queue);
     (void)hipFree(P->d_evals);
+    (void)hipFree(P->d_xfer);
     delete P;
     return nullptr;
   }
@@ -421,6 +428,31 @@ vj_fint vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream)
   return 0;
 }
 
+vj_fint vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream)
+{
+  const BPlan *P = (const BPlan *)plan;
+  if (!P || !elems)
+    return set_error(-1, "null plan or output");
+  hipStream_t s = (hipStream_t)stream;
+  if (P->nb)
+    VJ_HIP(hipMemcpyAsync(elems, P->d_xfer, (size_t)P->nb * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  VJ_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+vj_fint vjh_bplan_order(void *plan, int id)
+{
+  BPlan *P = (BPlan *)plan;
+  if (!P)
+    return set_error(-1, "null plan");
+  if (id == 0)
+    return P->ord;
+  if (id != 4 && id != 6)
+    return set_error(-2, "batched SVD: ordering %d (pjs id 4 or 6)", id);
+  P->ord = id;
+  return id;
+}
+
 vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
                            void *fS, vj_fint *info, float *ms, void *stream)
 {
@@ -505,6 +537,7 @@ void vjh_bplan_destroy(void *plan)
   (void)hipFree(P->d_order);
   (void)hipFree(P->d_queue);
   (void)hipFree(P->d_evals);
+  (void)hipFree(P->d_xfer);
   delete P;
 }
 

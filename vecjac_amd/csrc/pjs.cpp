@@ -12,9 +12,12 @@
 //        size h instead plays a round robin with byes (round r pairs
 //        i + j = 2r mod h and leaves r out) while the two byes A_r, B_r pair
 //        across, and the remaining h - 1 shifts t = 1..h-1 come first;
-//   id 4 (PJS_MM, n steps): the modulus ordering (mod_pair in pjs.hpp).
+//   id 4 (PJS_MM, n steps): the modulus ordering (mod_pair in pjs.hpp);
+//   id 6 (n steps, not in the reference): the block modulus ordering
+//        (br_pair in pjs.hpp), the batched kernel's default.
 // Every pair meets exactly once per sweep under id 2 (the n/2 pairs
-// {i, i + n/2} twice under id 4); every row is a permutation of 0..n-1
+// {i, i + n/2} twice under id 4, the n/2 pairs {2i, 2i + 1} twice under
+// id 6); every row is a permutation of 0..n-1
 // (needed for race-free column updates); each pair is written with its
 // smaller index first.
 #include "pjs.hpp"
@@ -75,12 +78,12 @@ void mantharam_eberlein(unsigned n, unsigned *tab)
   me_fill(idx.data(), n, tab, n, 0u, 0u);
 }
 
-void modulus(unsigned n, unsigned *tab)
+void computed(int id, unsigned n, unsigned *tab)
 {
   for (unsigned s = 0u; s < n; ++s)
     for (unsigned k = 0u; k < (n >> 1u); ++k) {
       unsigned *const r = tab + (size_t)s * n + 2u * k;
-      vj::mod_pair(n, s, k, r[0], r[1]);
+      vj::step_pair(id, n, s, k, r[0], r[1]);
     }
 }
 
@@ -97,6 +100,7 @@ extern "C" unsigned *pjs(const long id, const unsigned n, unsigned *stp)
     s = n - 1u;
     break;
   case 4l:
+  case 6l:
     s = n;
     break;
   default:
@@ -108,7 +112,7 @@ extern "C" unsigned *pjs(const long id, const unsigned n, unsigned *stp)
   if (id == 2l)
     mantharam_eberlein(n, tab);
   else
-    modulus(n, tab);
+    computed((int)id, n, tab);
   *stp = s;
   return tab;
 }

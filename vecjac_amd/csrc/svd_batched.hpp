@@ -15,10 +15,12 @@ template <typename T> struct BatchArgs {
   long long *info;                // per matrix: sweeps, or a negative code
   unsigned long long *transforms; // per matrix: transformations (may be null)
   unsigned long long *evals;      // per matrix: pair evaluations (may be null)
+  unsigned long long *xfer;       // per matrix: elements of G and V read or written in memory (may be null)
   unsigned *queue;                // work counter, zero on entry
   unsigned swp;                   // sweep limit
   int lds_bytes;                  // LDS-resident form: bytes of G and V per workgroup (0: global)
   int group_lg;                   // lane-group form: 2^group_lg lanes per pair (4, 3 or 2)
+  int ord;                        // ordering of the pairs: pjs id 4 (modulus) or 6 (block modulus)
 };
 
 // Shapes of the lane-group form that are built: 16 lanes per pair with any

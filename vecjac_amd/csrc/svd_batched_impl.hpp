@@ -258,19 +258,20 @@ template <typename T> __device__ __forceinline__ PairDec<T> lane_dec(const PairD
 }
 
 // One step of the strategy for the pairs of one wave (pairs wave, wave + W,
-// wave + 2W, ... of the modulus table of width n, pjs id 4), GP pairs at a time:
+// wave + 2W, ... of step st of ordering ord, pjs id 4 or 6), GP pairs at a time:
 // load (the next group is prefetched), reduce the GP pairs together, decide
 // with lane r of every quad working on pair r, broadcast each decision and
 // apply it in registers, store.  Accumulates this wave's per-lane max of the
 // updated G elements, its transformation count and its first error.
 template <typename T, bool CPLX, int E, int W, int GP, bool PF>
 __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n, const int nE,
-                                          const unsigned st, const int wave, const int lane, const Step<T> &S, T &wG,
-                                          unsigned long long &wcnt, int &werr, int *mod, const int gstep,
-                                          unsigned long long &wev)
+                                          const int ord, const unsigned st, const int wave, const int lane,
+                                          const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr, int *mod,
+                                          const int gstep, unsigned long long &wev, unsigned long long &wxf)
 {
   typedef typename Acc<T>::A A;
   constexpr int B = 64;
+  constexpr unsigned long long PL = CPLX ? 2ull : 1ull; // planes
   const int npairs = n >> 1;
   if (wave >= npairs)
     return;
@@ -285,7 +286,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
     bool need = false;
     if (lane < cnt0) {
       unsigned pp, qq;
-      mod_pair((unsigned)n, st, (unsigned)(wave + W * lane), pp, qq);
+      step_pair(ord, (unsigned)n, st, (unsigned)(wave + W * lane), pp, qq);
       need = (mod[pp] >= gstep - n) || (mod[qq] >= gstep - n);
     }
     const unsigned long long mk = __ballot(need);
@@ -297,6 +298,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
     slot = __builtin_amdgcn_ds_permute(pos << 2, lane);
   }
   wev += (unsigned long long)cnt;
+  wxf += (unsigned long long)cnt * 2ull * PL * (unsigned long long)(m + n);
   const int ngr = (cnt + GP - 1) / GP;
   T xr[GP][E], yr[GP][E], xi[GP][E], yi[GP][E], vx[GP][2], vy[GP][2], vxi[GP][2], vyi[GP][2];
   unsigned p[GP], q[GP];
@@ -305,7 +307,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
     int i = g * GP + r;
     i = (i < cnt) ? i : (cnt - 1);
     i = __builtin_amdgcn_readlane(slot, i);
-    mod_pair((unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
+    step_pair(ord, (unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
   };
   if constexpr (PF) {
 #pragma unroll
@@ -400,6 +402,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
         store_pair<T, CPLX, E>(Gr, Gi, m, nE, (int)p[r], (int)q[r], lane, xr[r], yr[r], xi[r], yi[r]);
         if (lane == 0)
           mod[p[r]] = mod[q[r]] = gstep;
+        wxf += 2ull * PL * (unsigned long long)m;
       }
       if (S.sV || (!D.err && (D.code != 1) && (D.code != 3))) {
         if (S.sV)
@@ -407,6 +410,7 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
         if (!D.err)
           v_update<T, CPLX>(D, vx[r], vy[r], vxi[r], vyi[r]);
         store_vpair<T, CPLX>(Vr, Vi, n, (int)p[r], (int)q[r], lane, vx[r], vy[r], vxi[r], vyi[r]);
+        wxf += 2ull * PL * (unsigned long long)n;
       }
       if (err)
         werr = err;
@@ -433,6 +437,242 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
       for (int j = 0; j < 2; ++j) {
         vx[r][j] = nvx[r][j]; vy[r][j] = nvy[r][j]; vxi[r][j] = nvxi[r][j]; vyi[r][j] = nvyi[r][j];
       }
+    }
+  }
+}
+
+// ------------------------------------------------------------ block rounds
+// Block form of the wave steps (real data, block modulus ordering, pjs id
+// 6): a wave takes whole units of a round (a block pair: four columns; or a
+// block met by itself: two), loads their G and V columns once, runs both
+// steps of the round on them in registers (the two pairs of a step reduced
+// and decided together, as in wave_step), and stores each changed column
+// once: half the column traffic and half the barriers of two wave_step
+// calls, with the results of those two steps.  (Between the two steps of a
+// round the workgroup folds nothing: the fold only acts through a rescaling
+// of G or V, which cannot occur in a batched run of real data, see k_bsvd.)
+
+// Column c of a rows-row matrix from registers (row lane + 64 j).
+template <typename T, int E>
+__device__ __forceinline__ void store_col(T *A, const int rows, const int nE, const unsigned c, const int lane,
+                                          const T (&x)[E])
+{
+  T *const Ac = A + (long long)c * rows;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
+    const int i = lane + j * 64;
+    if (i < rows)
+      Ac[i] = x[j];
+  }
+}
+
+// The two pairs (x0, y0) = columns (p0, q0) and (x1, y1) = (p1, q1) of one
+// step, reduced and decided together (lane r & 1 decides pair r) and updated
+// in place with their V columns; pair 1 is only applied when `two`.  Returns
+// bit r (r = 0, 1) set when pair r changed its G columns, bit 2 + r when it
+// changed its V columns; an error leaves everything unchanged from that pair
+// on and is returned in werr.
+template <typename T, int E>
+__device__ __forceinline__ unsigned pair2(T (&x0)[E], T (&y0)[E], T (&x1)[E], T (&y1)[E], T (&v0)[2], T (&w0)[2],
+                                          T (&v1)[2], T (&w1)[2], const bool two, const unsigned p0, const unsigned q0,
+                                          const unsigned p1, const unsigned q1, const int nE, const int lane,
+                                          const Step<T> &S, T *eS, T *fS, T &wG, unsigned long long &wcnt, int &werr)
+{
+  typedef typename Acc<T>::A A;
+  constexpr int B = 64;
+  T zi[E], vzi[2]; // imaginary parts of real data
+#pragma unroll
+  for (int j = 0; j < E; ++j)
+    zi[j] = (T)0;
+  vzi[0] = vzi[1] = (T)0;
+  T mx0, my0, mx1, my1;
+  pair_maxima<T, false, E>(x0, y0, zi, zi, nE, mx0, my0);
+  pair_maxima<T, false, E>(x1, y1, zi, zi, nE, mx1, my1);
+  red_max4(mx0, my0, mx1, my1);
+  const int kx0 = max_exp(mx0), ky0 = max_exp(my0), kx1 = max_exp(mx1), ky1 = max_exp(my1);
+  DD<A> sx0{0, 0}, sy0{0, 0}, dr0{0, 0}, di0{0, 0}, sx1{0, 0}, sy1{0, 0}, dr1{0, 0}, di1{0, 0};
+  pair_sums<T, false, E>(x0, y0, zi, zi, nE, kx0, ky0, sx0, sy0, dr0, di0);
+  red_dd4<A, B>(sx0, sy0, dr0, di0, nullptr);
+  pair_sums<T, false, E>(x1, y1, zi, zi, nE, kx1, ky1, sx1, sy1, dr1, di1);
+  red_dd4<A, B>(sx1, sy1, dr1, di1, nullptr);
+  const bool t1 = (lane & 1) != 0;
+  const PairScal<T> Pl = pair_scal<T, false>(t1 ? mx1 : mx0, t1 ? my1 : my0, t1 ? kx1 : kx0, t1 ? ky1 : ky0,
+                                             dd_sel(t1, sx1, sx0), dd_sel(t1, sy1, sy0), dd_sel(t1, dr1, dr0),
+                                             dd_sel(t1, di1, di0));
+  const PairDec<T> Dl = pair_decide<T, false>(Pl, S.tol, S.gst);
+  unsigned ch = 0u;
+  auto apply = [&](T (&x)[E], T (&y)[E], T (&v)[2], T (&w)[2], const int r, const unsigned p, const unsigned q) {
+    const PairDec<T> D = lane_dec(Dl, r);
+    const bool gsr = (D.code == 3) || (D.code == -3);
+    const PairScal<T> P = gsr ? lane_scal(Pl, r) : Pl;
+    T eSp, fSp, eSq, fSq;
+    const PairOut<T, false> o = pair_apply<T, false, B, E, false>(0, x, y, zi, zi, nullptr, nullptr, nullptr, nullptr,
+                                                                  P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane, nE);
+    int err = D.err;
+    if (!err && (__ballot(!isfinite_(o.tG)) != 0ull))
+      err = -580;
+    if (err) {
+      werr = err;
+      return;
+    }
+    if (D.code != 1)
+      ch |= 1u << r;
+    if ((D.code != 1) && (D.code != 3)) {
+      v_update<T, false>(D, v, w, vzi, vzi);
+      ch |= 4u << r;
+    }
+    if (lane == r) {
+      eS[p] = eSp; fS[p] = fSp;
+      eS[q] = eSq; fS[q] = fSq;
+    }
+    wG = xmax(wG, o.tG);
+    wcnt += (unsigned long long)o.trans;
+  };
+  apply(x0, y0, v0, w0, 0, p0, q0);
+  if (two && !werr)
+    apply(x1, y1, v1, w1, 1, p1, q1);
+  return ch;
+}
+
+// Register footprint of the block form: the next unit is prefetched while
+// its four G columns take at most 8 registers each.
+template <typename T, int E> struct BlockShape {
+  static constexpr bool PF = (E * (int)(sizeof(T) / 4)) <= 8;
+};
+
+// Block round s (steps 2s and 2s + 1, numbered g0 and g0 + 1 in the run) for
+// the units of one wave (units wave, wave + W, ...).  A unit none of whose
+// columns changed since step g0 - n (one sweep before) is skipped: each of
+// its pairs would repeat its no-op evaluation of the previous sweep.
+template <typename T, int E, int W, bool PF>
+__device__ __forceinline__ void block_round(T *Gr, T *Vr, T *eS, T *fS, const int m, const int n, const int nE,
+                                            const unsigned s, const int wave, const int lane, const Step<T> &S,
+                                            T &wG, unsigned long long &wcnt, int &werr, int *mod, const int g0,
+                                            unsigned long long &wev, unsigned long long &wxf)
+{
+  const unsigned nb = (unsigned)n >> 1u;
+  const int nu = (int)br_units(nb, s);
+  if (wave >= nu)
+    return;
+  const int cnt0 = (nu - wave + W - 1) / W;
+  int cnt = cnt0, slot = lane;
+  if ((S.sG == 0) && (S.sV == 0) && (g0 > n)) {
+    bool need = false;
+    if (lane < cnt0) {
+      unsigned I, J;
+      br_unit(nb, s, (unsigned)(wave + W * lane), I, J);
+      const int lim = g0 - n;
+      need = (mod[2u * I] >= lim) || (mod[2u * I + 1u] >= lim) || (mod[2u * J] >= lim) || (mod[2u * J + 1u] >= lim);
+    }
+    const unsigned long long mk = __ballot(need);
+    cnt = __popcll(mk);
+    if (cnt == 0)
+      return;
+    const int below = __popcll(mk & ((1ull << lane) - 1ull));
+    const int pos = need ? below : (cnt + lane - below);
+    slot = __builtin_amdgcn_ds_permute(pos << 2, lane);
+  }
+  auto unit_of = [&](const int i, unsigned &I, unsigned &J) {
+    const int k = __builtin_amdgcn_readlane(slot, (i < cnt) ? i : (cnt - 1));
+    br_unit(nb, s, (unsigned)(wave + W * k), I, J);
+  };
+  T zi[E], vzi[2];
+  auto load_unit = [&](const unsigned I, const unsigned J, T (&a)[E], T (&b)[E], T (&c)[E], T (&d)[E], T (&va)[2],
+                       T (&vb)[2], T (&vc)[2], T (&vd)[2]) {
+    load_pair<T, false, E>(Gr, nullptr, m, nE, (int)(2u * I), (int)(2u * I + 1u), lane, a, b, zi, zi);
+    load_pair<T, false, E>(Gr, nullptr, m, nE, (int)(2u * J), (int)(2u * J + 1u), lane, c, d, zi, zi);
+    load_vpair<T, false>(Vr, nullptr, n, (int)(2u * I), (int)(2u * I + 1u), lane, va, vb, vzi, vzi);
+    load_vpair<T, false>(Vr, nullptr, n, (int)(2u * J), (int)(2u * J + 1u), lane, vc, vd, vzi, vzi);
+  };
+  T ga[E], gb[E], gc[E], gd[E], va[2], vb[2], vc[2], vd[2];
+  unsigned I = 0u, J = 0u;
+  if constexpr (PF) {
+    unit_of(0, I, J);
+    load_unit(I, J, ga, gb, gc, gd, va, vb, vc, vd);
+  }
+  const int g1 = g0 + 1;
+  for (int i = 0; i < cnt; ++i) {
+    unsigned In = 0u, Jn = 0u;
+    T na[E], nbb[E], nc[E], nd[E], nva[2], nvb[2], nvc[2], nvd[2];
+    if constexpr (PF) {
+      unit_of((i + 1 < cnt) ? (i + 1) : i, In, Jn);
+      load_unit(In, Jn, na, nbb, nc, nd, nva, nvb, nvc, nvd);
+    }
+    else {
+      unit_of(i, I, J);
+      load_unit(I, J, ga, gb, gc, gd, va, vb, vc, vd);
+    }
+    ready_pair<T, false, E>(m, nE, S.sG, lane, ga, gb, zi, zi);
+    ready_pair<T, false, E>(m, nE, S.sG, lane, gc, gd, zi, zi);
+    if (S.sV) {
+      scale_vpair<T>(S.sV, va, vb, vzi, vzi);
+      scale_vpair<T>(S.sV, vc, vd, vzi, vzi);
+    }
+    // a block met by itself runs its pair (2I, 2I+1) at both steps as pair
+    // 0: column 2I+1 moves to register column c for the first step and on
+    // to d for the second
+    const bool diag = (I == J);
+    if (diag) {
+#pragma unroll
+      for (int j = 0; j < E; ++j)
+        gc[j] = gb[j];
+      vc[0] = vb[0];
+      vc[1] = vb[1];
+    }
+    const unsigned cA = 2u * I, cB = 2u * I + 1u, cC = 2u * J, cD = 2u * J + 1u;
+    const unsigned ch0 = pair2<T, E>(ga, gc, gb, gd, va, vc, vb, vd, !diag, cA, diag ? cB : cC, cB, cD, nE, lane, S,
+                                     eS, fS, wG, wcnt, werr);
+    if (werr)
+      return;
+    if (diag) {
+#pragma unroll
+      for (int j = 0; j < E; ++j)
+        gd[j] = gc[j];
+      vd[0] = vc[0];
+      vd[1] = vc[1];
+    }
+    const unsigned ch1 = pair2<T, E>(ga, gd, gb, gc, va, vd, vb, vc, !diag, cA, diag ? cB : cD, cB, cC, nE, lane, S,
+                                     eS, fS, wG, wcnt, werr);
+    if (werr)
+      return;
+    wev += diag ? 2ull : 4ull;
+    // changes per register column: a by pair 0 at both steps, b by pair 1
+    // at both, c by pair 0 then 1, d by pair 1 then 0 (for a block met by
+    // itself d holds column 2I+1, changed by pair 0 at both steps)
+    const unsigned s0a = ch0 & 5u, s1a = ch1 & 5u, s0b = (ch0 >> 1) & 5u, s1b = (ch1 >> 1) & 5u;
+    const unsigned chA[4] = {s0a | s1a, s0b | s1b, s0a | s1b, diag ? (s0a | s1a) : (s0b | s1a)};
+    const bool last1[4] = {(s1a & 1u) != 0u, (s1b & 1u) != 0u, (s1b & 1u) != 0u, (s1a & 1u) != 0u};
+    const unsigned col[4] = {cA, cB, cC, diag ? cB : cD};
+    unsigned long long xf = (unsigned long long)(diag ? 2 : 4) * (unsigned long long)(m + n);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (diag && (k == 1 || k == 2))
+        continue;
+      T (&g)[E] = (k == 0) ? ga : (k == 1) ? gb : (k == 2) ? gc : gd;
+      T (&v)[2] = (k == 0) ? va : (k == 1) ? vb : (k == 2) ? vc : vd;
+      if ((chA[k] & 1u) || S.sG) {
+        store_col<T, E>(Gr, m, nE, col[k], lane, g);
+        if (lane == 0)
+          mod[col[k]] = last1[k] ? g1 : g0;
+        xf += (unsigned long long)m;
+      }
+      if ((chA[k] & 4u) || S.sV) {
+        store_col<T, 2>(Vr, n, 2, col[k], lane, v);
+        xf += (unsigned long long)n;
+      }
+    }
+    wxf += xf;
+    if constexpr (PF) {
+      I = In;
+      J = Jn;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        ga[j] = na[j]; gb[j] = nbb[j]; gc[j] = nc[j]; gd[j] = nd[j];
+      }
+      va[0] = nva[0]; va[1] = nva[1]; vb[0] = nvb[0]; vb[1] = nvb[1];
+      vc[0] = nvc[0]; vc[1] = nvc[1]; vd[0] = nvd[0]; vd[1] = nvd[1];
     }
   }
 }
@@ -668,24 +908,27 @@ template <typename T, bool CPLX, int E> struct GroupShape {
 };
 
 // One step of the strategy for the pairs of one wave (pairs wave, wave + W,
-// wave + 2W, ... of the modulus table of width n, pjs id 4), P pairs at a
+// wave + 2W, ... of step st of ordering ord, pjs id 4 or 6), P pairs at a
 // time in lane groups: load (the next P prefetched when PF), reduce, decide
 // per lane, update in registers, store.  Accumulates this wave's per-lane
 // max of the updated G elements, its transformation count and its first
 // error.
 template <typename T, bool CPLX, int E, int W, bool PF, int LG>
 __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
-                                           const unsigned st, const int wave, const int lane,
+                                           const int ord, const unsigned st, const int wave, const int lane,
                                            const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr,
-                                           unsigned long long &wev)
+                                           unsigned long long &wev, unsigned long long *wxf)
 {
   typedef typename Acc<T>::A A;
   constexpr int lg = LG, LP = 1 << LG, P = 64 >> LG; // lanes per group, groups per wave
+  constexpr unsigned long long PL = CPLX ? 2ull : 1ull; // planes
   const int npairs = n >> 1;
   if (wave >= npairs)
     return;
   const int cnt = (npairs - wave + W - 1) / W; // pairs of this wave
   wev += (unsigned long long)cnt;
+  if (wxf)
+    *wxf += (unsigned long long)cnt * 2ull * PL * (unsigned long long)(m + n);
   const int nit = (cnt + P - 1) >> (6 - lg);
   const int g = lane >> lg, t = lane & (LP - 1);
   const int nE = (m + LP - 1) >> lg, nEV = (n + LP - 1) >> lg;
@@ -695,7 +938,7 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
     int i = it * P + g;
     const bool v = (i < cnt);
     i = v ? i : (cnt - 1);
-    mod_pair((unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
+    step_pair(ord, (unsigned)n, st, (unsigned)(wave + W * i), pp, qq);
     return v;
   };
   T xr[E], yr[E], xi[E], yi[E], vx[E], vy[E], vxi[E], vyi[E];
@@ -806,10 +1049,16 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
     if (!err && ((__ballot(!isfinite_(tG)) & gmask) != 0ull))
       err = CPLX ? -630 : -580;
     const unsigned dp = dsw ? q : p, dq = dsw ? p : q;
-    if (valid && !err && ((code != 1) || S.sG))
+    const bool stG = valid && !err && ((code != 1) || S.sG);
+    const bool stV = valid && (S.sV || (ok && (code != 1) && (code != 3)));
+    if (stG)
       store_cols<T, CPLX, E>(Gr, Gi, m, nE, dp, dq, t, LP, xr, yr, xi, yi);
-    if (valid && (S.sV || (ok && (code != 1) && (code != 3))))
+    if (stV)
       store_cols<T, CPLX, E>(Vr, Vi, n, nEV, dp, dq, t, LP, vx, vy, vxi, vyi);
+    if (wxf)
+      *wxf += 2ull * PL *
+              ((unsigned long long)__popcll(__ballot(stG && (t == 0))) * (unsigned long long)m +
+               (unsigned long long)__popcll(__ballot(stV && (t == 0))) * (unsigned long long)n);
     // (e,f) of the pair as pair_apply leaves them: exchanged for -1 / -3
     if (valid && !err && (t == 0)) {
       const bool xe = (code == -1) || (code == -3);
@@ -846,52 +1095,87 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
 // several sums over the wave.)
 template <typename T, bool CPLX, int E, int W, int LG>
 __device__ __forceinline__ void group_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
-                                               const unsigned st, const int wave, const int lane, const Step<T> &S,
-                                               T &wG, unsigned long long &wcnt, int &werr, unsigned long long &wev)
+                                               const int ord, const unsigned st, const int wave, const int lane,
+                                               const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr,
+                                               unsigned long long &wev, unsigned long long *wxf)
 {
   constexpr int E64 = 64 >> LG; // chunks of 2^LG rows for 64 rows
   if ((E < 2) || CPLX || (m <= 64)) {
-    group_step<T, CPLX, E64, W, GroupShape<T, CPLX, E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, st, wave, lane, S, wG,
-                                                                  wcnt, werr, wev);
+    group_step<T, CPLX, E64, W, GroupShape<T, CPLX, E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, ord, st, wave, lane, S,
+                                                                  wG, wcnt, werr, wev, wxf);
     return;
   }
   if constexpr ((E >= 2) && !CPLX && (2 * E64 <= 16))
-    group_step<T, CPLX, 2 * E64, W, GroupShape<T, CPLX, 2 * E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, st, wave, lane,
-                                                                          S, wG, wcnt, werr, wev);
+    group_step<T, CPLX, 2 * E64, W, GroupShape<T, CPLX, 2 * E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, ord, st, wave,
+                                                                          lane, S, wG, wcnt, werr, wev, wxf);
 }
 
 // Longer columns: a pair per 64 lanes, two pairs per wave, with the register
 // footprint (64-row chunks per lane) of this matrix's row count.
 template <typename T, bool CPLX, int E, int W>
 __device__ __forceinline__ void wave_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
-                                              const unsigned st, const int wave, const int lane, const Step<T> &S,
-                                              T &wG, unsigned long long &wcnt, int &werr, int *mod, const int gstep,
-                                              unsigned long long &wev)
+                                              const int ord, const unsigned st, const int wave, const int lane,
+                                              const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr, int *mod,
+                                              const int gstep, unsigned long long &wev, unsigned long long &wxf)
 {
   const int nE = (m + 63) >> 6;
+#define VJ_WS(EE)                                                                                                     \
+  wave_step<T, CPLX, EE, W, StepShape<T, CPLX, EE>::GP, StepShape<T, CPLX, EE>::PF>(                                   \
+      Gr, Gi, Vr, Vi, eS, fS, m, n, nE, ord, st, wave, lane, S, wG, wcnt, werr, mod, gstep, wev, wxf)
   if ((E < 4) || (nE <= 2)) {
     constexpr int E2 = (E >= 2) ? 2 : E;
-    wave_step<T, CPLX, E2, W, StepShape<T, CPLX, E2>::GP, StepShape<T, CPLX, E2>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
-                                                                                      wave, lane, S, wG, wcnt, werr, mod, gstep, wev);
+    VJ_WS(E2);
     return;
   }
   if constexpr (E >= 4) {
     if ((E < 8) || (nE <= 4)) {
-      wave_step<T, CPLX, 4, W, StepShape<T, CPLX, 4>::GP, StepShape<T, CPLX, 4>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
-                                                                                     wave, lane, S, wG, wcnt, werr, mod, gstep, wev);
+      VJ_WS(4);
       return;
     }
   }
   if constexpr (E >= 8) {
     if ((E < 16) || (nE <= 8)) {
-      wave_step<T, CPLX, 8, W, StepShape<T, CPLX, 8>::GP, StepShape<T, CPLX, 8>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
-                                                                                     wave, lane, S, wG, wcnt, werr, mod, gstep, wev);
+      VJ_WS(8);
       return;
     }
   }
   if constexpr (E >= 16)
-    wave_step<T, CPLX, E, W, StepShape<T, CPLX, E>::GP, StepShape<T, CPLX, E>::PF>(Gr, Gi, Vr, Vi, eS, fS, m, n, nE, st,
-                                                                                   wave, lane, S, wG, wcnt, werr, mod, gstep, wev);
+    VJ_WS(E);
+#undef VJ_WS
+}
+
+// Block rounds of the wave form (real data, pjs id 6), with the register
+// footprint of this matrix's row count.
+template <typename T, int E, int W>
+__device__ __forceinline__ void block_dispatch(T *Gr, T *Vr, T *eS, T *fS, const int m, const int n, const unsigned s,
+                                               const int wave, const int lane, const Step<T> &S, T &wG,
+                                               unsigned long long &wcnt, int &werr, int *mod, const int g0,
+                                               unsigned long long &wev, unsigned long long &wxf)
+{
+  const int nE = (m + 63) >> 6;
+#define VJ_BR(EE)                                                                                                     \
+  block_round<T, EE, W, BlockShape<T, EE>::PF>(Gr, Vr, eS, fS, m, n, nE, s, wave, lane, S, wG, wcnt, werr, mod, g0,  \
+                                               wev, wxf)
+  if ((E < 4) || (nE <= 2)) {
+    constexpr int E2 = (E >= 2) ? 2 : E;
+    VJ_BR(E2);
+    return;
+  }
+  if constexpr (E >= 4) {
+    if ((E < 8) || (nE <= 4)) {
+      VJ_BR(4);
+      return;
+    }
+  }
+  if constexpr (E >= 8) {
+    if ((E < 16) || (nE <= 8)) {
+      VJ_BR(8);
+      return;
+    }
+  }
+  if constexpr (E >= 16)
+    VJ_BR(E);
+#undef VJ_BR
 }
 
 // Minimum waves per SIMD the register allocation must allow: two, which keeps
@@ -918,6 +1202,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
   __shared__ T s_eS[128], s_fS[128];           // the matrix's (e,f) column norms (n <= 128)
   __shared__ int s_mod[128];                   // step at which each column was last modified
   __shared__ unsigned long long s_ev;          // pair evaluations of the matrix
+  __shared__ unsigned long long s_xf;          // elements of G and V moved to and from memory
   __shared__ double s_max[2][W][2];            // per-wave (tG, tV) maxima, by step parity
   __shared__ unsigned long long s_cnt[2][W];   // per-wave transformation counts, by step parity
   __shared__ int s_err[2][W];
@@ -946,10 +1231,21 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     T *const Vi = CPLX ? (LDS ? lV + n * n : gVi) : nullptr;
     T *const eS = a.eS + a.offS[b];
     T *const fS = a.fS + a.offS[b];
-    const unsigned stp = (unsigned)n; // steps per sweep of the modulus ordering
+    const unsigned stp = (unsigned)n; // steps per sweep (pjs id 4 or 6)
     unsigned long long wev = 0ull;    // this wave's pair evaluations
-    if (threadIdx.x == 0)
+    unsigned long long wxf = 0ull;    // this wave's elements moved (global-memory forms)
+    if (threadIdx.x == 0) {
       s_ev = 0ull;
+      s_xf = 0ull;
+    }
+    // The block form runs the two steps of a round without a fold between
+    // them, which equals the step-by-step run as long as no step rescales G
+    // or V.  For real data none can: G starts scaled to max |g| < 2^NRM_EXP
+    // <= M/(2m) (M the largest finite value), the update maxima are bounded
+    // by |g_ij| <= |G|_F <= sqrt(m n) max |g| < M/2, i.e. below 2^(ROT_EXP_R
+    // + 1) with a margin of at least 2^-10 relative (m <= 1024), which the
+    // rounding of the rotations cannot close; V stays orthogonal.
+    const bool blk = !GRP && !CPLX && (a.ord == 6);
 
     // ---- setup (dvjsvd.c:78-184): max |G|, V := I, eS/fS, initial scale
     T mxv = (T)0;
@@ -992,6 +1288,9 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       }
     }
     const bool norms = !status; // eS/fS are written back (from LDS) at the end
+    // elements the global-memory forms move outside the steps: the scan of
+    // G, V := I, and below the initial scaling and the epilogue
+    unsigned long long xset = (unsigned long long)m * (unsigned long long)n + (norms ? (unsigned long long)n * n : 0ull);
     unsigned sw = 0;
     unsigned long long total = 0;
     int sT = 0, NRM_EXP = 0;
@@ -1009,6 +1308,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           if constexpr (CPLX)
             Gi[t] = ldexp_(Gi[t], sN);
         }
+        xset += 2ull * (unsigned long long)m * (unsigned long long)n;
       }
       __syncthreads();
       Step<T> S;
@@ -1026,11 +1326,21 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           unsigned long long wcnt = 0;
           int werr = 0;
           if constexpr (GRP)
-            group_dispatch<T, CPLX, E, W, LG>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr,
-                                              wev);
-          else
-            wave_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, st, wave, lane, S, wG, wcnt, werr, s_mod,
-                                         gstep, wev);
+            group_dispatch<T, CPLX, E, W, LG>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, a.ord, st, wave, lane, S, wG, wcnt,
+                                              werr, wev, LDS ? nullptr : &wxf);
+          else {
+            if constexpr (!CPLX) {
+              if (blk) {
+                block_dispatch<T, E, W>(Gr, Vr, s_eS, s_fS, m, n, st >> 1, wave, lane, S, wG, wcnt, werr, s_mod,
+                                        gstep, wev, wxf);
+                ++gstep;
+                ++st;
+              }
+            }
+            if (!blk)
+              wave_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, a.ord, st, wave, lane, S, wG, wcnt, werr,
+                                           s_mod, gstep, wev, wxf);
+          }
           wG = red_max<T, 64>(wG, nullptr);
           if (lane == 0) {
             s_max[par][wave][0] = (double)wG;
@@ -1094,6 +1404,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
         __syncthreads();
         for (int j = threadIdx.x; j < n; j += 64 * W)
           s_eS[j] -= (T)sT;
+        xset += 2ull * (unsigned long long)m * (unsigned long long)n;
       }
     }
     if constexpr (LDS) {
@@ -1114,6 +1425,8 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     }
     if (lane == 0 && wev)
       atomicAdd(&s_ev, wev);
+    if (lane == 0 && wxf)
+      atomicAdd(&s_xf, wxf);
     __syncthreads();
     if (norms)
       for (int j = threadIdx.x; j < n; j += 64 * W) {
@@ -1126,6 +1439,11 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
         a.transforms[b] = total;
       if (a.evals)
         a.evals[b] = s_ev;
+      if (a.xfer) {
+        // the LDS-resident form moves G in, and U (or G) and V out, once
+        const unsigned long long pl = CPLX ? 2ull : 1ull;
+        a.xfer[b] = LDS ? pl * (unsigned long long)n * (unsigned long long)(norms ? (2 * m + n) : m) : (s_xf + pl * xset);
+      }
     }
   }
 }
