@@ -43,6 +43,20 @@ def _host_has_avx512() -> bool:
         return False
 
 
+def _profiled_valu(name: str):
+    """VALU issue utilisation of a kernel on its default workload, from the
+    rocprofv3 SQ counters recorded in profiles/<name>.md on a line
+    `valu_busy: <fraction>`.  None when absent."""
+    f = ROOT / "profiles" / f"{name}.md"
+    try:
+        for line in f.read_text().splitlines():
+            if line.strip().startswith("valu_busy:"):
+                return float(line.split(":", 1)[1].strip().split()[0])
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
+
+
 def _profiled_traffic(name: str):
     """HBM bytes per launch of a kernel on its default workload, as measured
     by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, each scaled by the
@@ -261,6 +275,8 @@ def bench_bsvd(args, kind: str = "d"):
         out["roofline"]["traffic"] = t
         if src:
             out["roofline"]["traffic_source"] = src
+            # the long-column launch is bound by VALU issue, not by HBM
+            out["roofline"]["valu_busy"] = _profiled_valu(f"k_bsvd_{'fp64' if kind == 'd' else 'fp32'}_16384")
     # sigma accuracy on a sample, against the oracle on the same matrices
     if not args.no_check:
         out["sigma_check"] = _bsvd_sigma_check(m, npad, offG, offS, G64, r, kind)

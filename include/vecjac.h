@@ -93,6 +93,16 @@ vj_fint dvjsvd_(const vj_fnat *m, const vj_fnat *n, double *G, const vj_fnat *ld
  * it is switched off again, so a caller switches it on around the calls it
  * traces only.  Off by default. */
 void vjh_set_jtrace(int on);
+/* Rotation form switch (the reference's compile-time USE_SECANTS,
+ * vecdef.h:48-52,97-101, dvjsvd.c:411-415): with on != 0 the plain drivers
+ * (?vjsvd_, vjh_?vjsvd, vjh_dvjsvd_noalloc) take secants from the 2x2 EVD
+ * (the internal form ?bjac2i with n < 0) and every rotation divides by the
+ * secant instead of multiplying by the cosine, as a reference built with
+ * -DUSE_SECANTS does.  Process-wide and sticky; off by default (the
+ * reference's default build).  The batched SVD and the LAPACK-style drivers
+ * always use cosines. */
+void vjh_set_secants(int on);
+int vjh_get_secants(void);
 
 /* Complex (split re/im) twin; replaces zvjsvd_ (zvjsvd.h:10, zvjsvd.c:40). */
 vj_fint zvjsvd_(const vj_fnat *m, const vj_fnat *n, double *Gr, const vj_fnat *ldGr, double *Gi, const vj_fnat *ldGi,

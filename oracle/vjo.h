@@ -61,6 +61,10 @@ float vjo_sgsscl(vjo_fint m, float t, float *x, float *y, const float *e, const 
  * dnormf_ (default), 1 = the GPU's correctly rounded scaled norm */
 void vjo_set_norm_mode(int mode);
 int vjo_get_norm_mode(void);
+/* rotation form used by the drivers: 0 = cosines (default), 1 = secants (the
+ * reference's USE_SECANTS build, vecdef.h:97-101, dvjsvd.c:411-415) */
+void vjo_set_secants(int on);
+int vjo_get_secants(void);
 
 /* SVD drivers: reference semantics (dvjsvd.h:10, zvjsvd.h:10, svjsvd.h:10, cvjsvd.h:10) */
 vjo_fint vjo_dvjsvd(const vjo_fnat *m, const vjo_fnat *n, double *G, const vjo_fnat *ldG, double *V, const vjo_fnat *ldV, double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work, unsigned *iwork);

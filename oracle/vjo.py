@@ -72,6 +72,8 @@ def oracle() -> C.CDLL:
         L.vjo_sdpscl.restype, L.vjo_sdpscl.argtypes = C.c_float, [_fnat, _vp, _vp, _vp, _vp]
         L.vjo_set_norm_mode.restype, L.vjo_set_norm_mode.argtypes = None, [C.c_int]
         L.vjo_get_norm_mode.restype, L.vjo_get_norm_mode.argtypes = C.c_int, []
+        L.vjo_set_secants.restype, L.vjo_set_secants.argtypes = None, [C.c_int]
+        L.vjo_get_secants.restype, L.vjo_get_secants.argtypes = C.c_int, []
         for nm, t in (("vjo_dvjsvd_batch", C.c_double), ("vjo_svjsvd_batch", C.c_float)):
             f = getattr(L, nm)
             f.restype = C.c_int64
@@ -80,32 +82,49 @@ def oracle() -> C.CDLL:
     return _oracle
 
 
+_ref_sec = None
+
+
 def ref() -> C.CDLL | None:
     """The reference's own kernels (built from its sources), or None."""
     global _ref
     if _ref is None and REF_SO.exists():
-        L = C.CDLL(str(REF_SO))
-        P = C.POINTER
-        for nm in ("dbjac2_", "dbjac2i", "sbjac2_", "sbjac2i"):
-            f = getattr(L, nm)
-            f.restype, f.argtypes = _fint, [P(_fint)] + [_vp] * 8
-        for nm in ("zbjac2_", "zbjac2i", "cbjac2_", "cbjac2i"):
-            f = getattr(L, nm)
-            f.restype, f.argtypes = _fint, [P(_fint)] + [_vp] * 10
-        L.ddpscl_.restype, L.ddpscl_.argtypes = C.c_double, [P(_fnat), _vp, _vp, _vp, _vp]
-        L.sdpscl_.restype, L.sdpscl_.argtypes = C.c_float, [P(_fnat), _vp, _vp, _vp, _vp]
-        L.zdpscl_.restype, L.zdpscl_.argtypes = Cplx, [P(_fnat), _vp, _vp, _vp, _vp, _vp, _vp]
-        L.dnormf_.restype, L.dnormf_.argtypes = C.c_double, [P(_fnat), _vp, _vp, _vp, _vp, _vp]
-        L.djrot_.restype, L.djrot_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp]
-        L.djrotf_.restype, L.djrotf_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp]
-        L.zjrot_.restype, L.zjrot_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp, _vp, _vp, _vp]
-        L.dgsscl_.restype, L.dgsscl_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp, _vp]
-        L.zgsscl_.restype, L.zgsscl_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
-        L.dvjsvd_.restype = _fint
-        L.dvjsvd_.argtypes = [P(_fnat), P(_fnat), _vp, P(_fnat), _vp, P(_fnat), _vp, _vp, _vp, P(C.c_uint),
-                              P(C.c_uint), _vp, _vp]
-        _ref = L
+        _ref = _declare_ref(C.CDLL(str(REF_SO)))
     return _ref
+
+
+def ref_sec() -> C.CDLL | None:
+    """The reference built with USE_SECANTS (its rotations and dvjsvd_), or
+    None."""
+    global _ref_sec
+    so = HERE / "_ref" / "libvjref_sec.so"
+    if _ref_sec is None and so.exists():
+        _ref_sec = _declare_ref(C.CDLL(str(so)))
+    return _ref_sec
+
+
+def _declare_ref(L: C.CDLL) -> C.CDLL:
+    """Prototypes of the reference build's entry points."""
+    P = C.POINTER
+    for nm in ("dbjac2_", "dbjac2i", "sbjac2_", "sbjac2i"):
+        f = getattr(L, nm)
+        f.restype, f.argtypes = _fint, [P(_fint)] + [_vp] * 8
+    for nm in ("zbjac2_", "zbjac2i", "cbjac2_", "cbjac2i"):
+        f = getattr(L, nm)
+        f.restype, f.argtypes = _fint, [P(_fint)] + [_vp] * 10
+    L.ddpscl_.restype, L.ddpscl_.argtypes = C.c_double, [P(_fnat), _vp, _vp, _vp, _vp]
+    L.sdpscl_.restype, L.sdpscl_.argtypes = C.c_float, [P(_fnat), _vp, _vp, _vp, _vp]
+    L.zdpscl_.restype, L.zdpscl_.argtypes = Cplx, [P(_fnat), _vp, _vp, _vp, _vp, _vp, _vp]
+    L.dnormf_.restype, L.dnormf_.argtypes = C.c_double, [P(_fnat), _vp, _vp, _vp, _vp, _vp]
+    L.djrot_.restype, L.djrot_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp]
+    L.djrotf_.restype, L.djrotf_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp]
+    L.zjrot_.restype, L.zjrot_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.dgsscl_.restype, L.dgsscl_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp, _vp]
+    L.zgsscl_.restype, L.zgsscl_.argtypes = C.c_double, [P(_fint), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.dvjsvd_.restype = _fint
+    L.dvjsvd_.argtypes = [P(_fnat), P(_fnat), _vp, P(_fnat), _vp, P(_fnat), _vp, _vp, _vp, P(C.c_uint),
+                          P(C.c_uint), _vp, _vp]
+    return L
 
 
 def aligned(shape, dtype, fill=0) -> np.ndarray:
@@ -193,6 +212,25 @@ class norm_mode:
 
     def __exit__(self, *exc):
         oracle().vjo_set_norm_mode(self.prev)
+        return False
+
+
+class secants:
+    """Context manager selecting the oracle's rotation form: on = the
+    reference's USE_SECANTS build (the 2x2 EVD returns secants and the
+    rotations divide by them), off = cosines (the default build)."""
+
+    def __init__(self, on: bool = True):
+        self.on = int(bool(on))
+
+    def __enter__(self):
+        L = oracle()
+        self.prev = L.vjo_get_secants()
+        L.vjo_set_secants(self.on)
+        return self
+
+    def __exit__(self, *exc):
+        oracle().vjo_set_secants(self.prev)
         return False
 
 

@@ -379,9 +379,9 @@ vjo_fint DRVNAME(const int strict, const vjo_fnat *const m, const vjo_fnat *cons
           unsigned P = 0u;
 #if CPLX
           const T w3 = SFX(scalef)(a21i[k], d);
-          SFX(cjac2)(VJO_TAN, w0, w1, w2, w3, cc + k, at + k, sat + k, l1 + k, l2 + k, &P);
+          SFX(cjac2)(vjo_secants ? VJO_SEC : VJO_TAN, w0, w1, w2, w3, cc + k, at + k, sat + k, l1 + k, l2 + k, &P);
 #else
-          SFX(rjac2)(VJO_TAN, w0, w1, w2, cc + k, at + k, l1 + k, l2 + k, &P);
+          SFX(rjac2)(vjo_secants ? VJO_SEC : VJO_TAN, w0, w1, w2, cc + k, at + k, l1 + k, l2 + k, &P);
 #endif
           /* action code (dvjsvd.c:438-447) */
           code[k] = gsp ? (T)3 : (gsn ? (T)-3 : (P ? (T)-2 : (T)2));

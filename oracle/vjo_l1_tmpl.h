@@ -240,7 +240,8 @@ void CN(dpscl)(const vjo_fnat m, const T *const xr, const T *const xi, const T *
 }
 
 /* djrot_ (djrot.c:5-107) / djrotf_ (djrotf.c:5-100); track selects the
- * max-|element| return of the tracking form. USE_SECANTS off: multiply by c. */
+ * max-|element| return of the tracking form.  c != 1 scales by c, or
+ * divides by it (the secant) in the USE_SECANTS form (VDMULDIV, vecdef.h:97-101). */
 static T RN(jrot_any)(const int track, const vjo_fint n, T *const x, T *const y, const T c, const T at)
 {
   if (!n)
@@ -259,7 +260,7 @@ static T RN(jrot_any)(const int track, const vjo_fint n, T *const x, T *const y,
       const T x_ = x[i], y_ = y[i];
       T x_r = FMA(-x_, at, y_);
       T y_r = FMA(y_, at, x_);
-      if (c != (T)1) { x_r = x_r * c; y_r = y_r * c; }
+      if (c != (T)1) { x_r = VJO_CSCALE(x_r, c); y_r = VJO_CSCALE(y_r, c); }
       mx = SFX(xmax)(mx, SFX(xmax)(SFX(abs)(x_r), SFX(abs)(y_r)));
       x[i] = x_r;
       y[i] = y_r;
@@ -273,7 +274,7 @@ static T RN(jrot_any)(const int track, const vjo_fint n, T *const x, T *const y,
       const T x_ = x[i], y_ = y[i];
       T x_r = FMA(y_, at, x_);
       T y_r = FMA(-x_, at, y_);
-      if (c != (T)1) { x_r = x_r * c; y_r = y_r * c; }
+      if (c != (T)1) { x_r = VJO_CSCALE(x_r, c); y_r = VJO_CSCALE(y_r, c); }
       mx = SFX(xmax)(mx, SFX(xmax)(SFX(abs)(x_r), SFX(abs)(y_r)));
       x[i] = x_r;
       y[i] = y_r;
@@ -314,11 +315,11 @@ static T CN(jrot_any)(const int track, const vjo_fint n, T *const xr, T *const x
     /* first VZFMA(_r,_i, y_r,y_i, cta,sta, x_r,x_i) */
     T ar = FMA(y_r, cat, FMA(-y_i, sat, x_r));
     T ai = FMA(y_r, sat, FMA(y_i, cat, x_i));
-    if (c != (T)1) { ar = ar * c; ai = ai * c; }
+    if (c != (T)1) { ar = VJO_CSCALE(ar, c); ai = VJO_CSCALE(ai, c); }
     /* second VZFMA(_r,_i, x_r,x_i, cna,sta, y_r,y_i) */
     T br = FMA(x_r, cna, FMA(-x_i, sat, y_r));
     T bi = FMA(x_r, sat, FMA(x_i, cna, y_i));
-    if (c != (T)1) { br = br * c; bi = bi * c; }
+    if (c != (T)1) { br = VJO_CSCALE(br, c); bi = VJO_CSCALE(bi, c); }
     mx = SFX(xmax)(mx, SFX(xmax)(SFX(abs)(ar), SFX(abs)(ai)));
     mx = SFX(xmax)(mx, SFX(xmax)(SFX(abs)(br), SFX(abs)(bi)));
     if (n < 0) { yr[i] = ar; yi[i] = ai; xr[i] = br; xi[i] = bi; }

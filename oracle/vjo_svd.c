@@ -26,6 +26,15 @@ static int vjo_norm_mode = 0;
 void vjo_set_norm_mode(int mode) { vjo_norm_mode = mode; }
 int vjo_get_norm_mode(void) { return vjo_norm_mode; }
 
+/* 0: the rotations scale by the cosine (the reference's default build);
+ * 1: the reference built with USE_SECANTS (vecdef.h:48-52,97-101): the
+ * drivers ask the 2x2 EVD for secants (dvjsvd.c:411-415, dbjac2i with
+ * n < 0) and the rotations divide by them */
+static int vjo_secants = 0;
+void vjo_set_secants(int on) { vjo_secants = on; }
+int vjo_get_secants(void) { return vjo_secants; }
+#define VJO_CSCALE(v, c) (vjo_secants ? ((v) / (c)) : ((v) * (c)))
+
 /* ------------------------------ double ------------------------------ */
 #define T double
 #define SFX(x) x##_d

@@ -61,8 +61,8 @@ def dgsex(N: int = 16) -> np.ndarray:
     return A
 
 
-def ref_dvjsvd(A: np.ndarray, swp: int = 999, js=None):
-    R = vjo.ref()
+def ref_dvjsvd(A: np.ndarray, swp: int = 999, js=None, R=None):
+    R = R or vjo.ref()
     n, m = A.shape
     if js is None:
         js, _ = S.pjs(2, n)

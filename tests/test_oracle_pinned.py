@@ -8,7 +8,9 @@
 2. Against the reference build itself, when oracle/_ref/libvjref.so is
    present: the level-1 kernels (ddpscl_/zdpscl_/sdpscl_, djrot_/djrotf_/
    zjrot_, dgsscl_/zgsscl_), the internal 2x2 forms and dvjsvd_ on fresh
-   random inputs.  Bar: bit-for-bit equality.
+   random inputs; and the rotations and dvjsvd_ of the reference built with
+   USE_SECANTS (oracle/_ref/libvjref_sec.so) against the oracle's secant
+   form.  Bar: bit-for-bit equality.
 """
 import ctypes as C
 from pathlib import Path
@@ -142,16 +144,23 @@ def test_dot_kernels_vs_ref(m):
             assert np.float32(a).view(np.uint32) == np.float32(b).view(np.uint32)
 
 
+sec_only = pytest.mark.skipif(vjo.ref_sec() is None, reason="USE_SECANTS reference build not present")
+
+
 @ref_only
+@pytest.mark.parametrize("sec", [False, pytest.param(True, marks=sec_only)])
 @pytest.mark.parametrize("n", [8, 64, 512])
-def test_rotation_and_gs_kernels_vs_ref(n):
-    R = vjo.ref()
+def test_rotation_and_gs_kernels_vs_ref(n, sec):
+    R = vjo.ref_sec() if sec else vjo.ref()
     L = vjo.oracle()
     g = W.rng(n + 1)
+    ctx = vjo.secants(sec)
+    ctx.__enter__()
     for trial in range(12):
         x0 = g.standard_normal(n) * np.ldexp(1.0, g.integers(-50, 50))
         y0 = g.standard_normal(n) * np.ldexp(1.0, g.integers(-50, 50))
-        c = [1.0, 1.0 / np.sqrt(1.0 + 0.3 ** 2), 0.5][trial % 3]
+        # a cosine (<= 1), or a secant (>= 1) in the USE_SECANTS form
+        c = [1.0, np.sqrt(1.0 + 0.3 ** 2), 2.0][trial % 3] if sec else [1.0, 1.0 / np.sqrt(1.0 + 0.3 ** 2), 0.5][trial % 3]
         t = [0.0, 0.3, -1.7][(trial // 3) % 3]
         for sgn in (1, -1):
             xs = [vjo.aligned(n, np.float64) for _ in range(2)]
@@ -195,6 +204,7 @@ def test_rotation_and_gs_kernels_vs_ref(n):
             r2 = R.dgsscl_(C.byref(C.c_int64(sgn * n)), C.byref(C.c_double(tcos)), xs2[1].ctypes.data,
                            ys2[1].ctypes.data, e.ctypes.data, f.ctypes.data)
             assert r1 == r2 and bit_mismatch(xs2[0], xs2[1]) == 0 and bit_mismatch(ys2[0], ys2[1]) == 0
+    ctx.__exit__(None, None, None)
 
 
 @ref_only
@@ -213,3 +223,26 @@ def test_dvjsvd_vs_ref(m, n, seed):
     assert rc == rc_r
     assert bit_mismatch(G, G_r) == 0 and bit_mismatch(V, V_r) == 0
     assert bit_mismatch(eS, eS_r) == 0 and bit_mismatch(fS, fS_r) == 0
+
+
+@sec_only
+@pytest.mark.parametrize("m,n,seed", [(64, 16, 1), (96, 64, 2), (200, 80, 3)])
+def test_dvjsvd_secants_vs_ref(m, n, seed):
+    """The reference's dvjsvd_ built with USE_SECANTS (the 2x2 EVD returns
+    secants, dvjsvd.c:411-415, and the rotations divide by them) against the
+    oracle's secant form, bit for bit; the two forms do differ."""
+    from tests.golden.make_golden import ref_dvjsvd
+    A = W.gaussian_matrix(m, n, seed)
+    if seed == 3:
+        A *= np.ldexp(1.0, (np.arange(n) * 11 - 400))[:, None]
+    rc_r, G_r, V_r, eS_r, fS_r = ref_dvjsvd(A, R=vjo.ref_sec())
+    rc_c, G_c, V_c, _, _ = ref_dvjsvd(A)
+    js, stp = _pjs_table(n)
+    G = vjo.aligned((n, m), np.float64)
+    G[...] = A
+    with vjo.norm_mode(0), vjo.secants(True):
+        rc, eS, fS, V, _, _ = vjo.vjsvd_core("d", G, js=js, stp=stp, swp=999, strict=True)
+    assert rc == rc_r
+    assert bit_mismatch(G, G_r) == 0 and bit_mismatch(V, V_r) == 0
+    assert bit_mismatch(eS, eS_r) == 0 and bit_mismatch(fS, fS_r) == 0
+    assert bit_mismatch(V_c, V_r) > 0

@@ -8,7 +8,10 @@ arithmetic, on the inputs the rest of the suite leaves out:
     1024 x 256), and the column-graded A = B D, whose small singular values
     are determined to high relative accuracy;
   * full-width square matrices (512^2 and the 1024^2 workload itself);
-  * a run resumed from a preset V (iwork[0] = 1, dvjsvd.c:86-96, 143-157).
+  * a run resumed from a preset V (iwork[0] = 1, dvjsvd.c:86-96, 143-157);
+  * the reference's USE_SECANTS rotation form (vjh_set_secants), against the
+    oracle's secant form, which is pinned to the reference built with
+    -DUSE_SECANTS.
 
 Every comparison runs the oracle in both norm modes on the same input and
 step table: mode 0 is the reference's arithmetic (bit-identical to the
@@ -311,3 +314,32 @@ def test_vpreset_zero_refused(kind):
         rc = S.vjsvd("z", m, n, G, m, V, n, eS, fS, js, stp, 999, work, iw, Gi=Gi, ldGi=m, Vi=Vi, ldVi=n)
     assert rc == code
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("kind,m,n", [("d", 300, 96), ("d", 1024, 256), ("z", 400, 64)])
+def test_secants_form(kind, m, n):
+    """With vjh_set_secants(1) the driver equals the oracle's secant form in
+    the GPU's arithmetic bit for bit, is within 8 n eps of the reference's
+    arithmetic (mode 0) on a well-conditioned input, and differs from the
+    cosine form (the switch acts)."""
+    g = W.rng(5150 + n)
+    A = g.standard_normal((m, n))
+    if kind == "z":
+        A = A + 1j * g.standard_normal((m, n))
+    Ar, Ai = _split(A)
+    js, _ = S.pjs(2, n)
+    with S.secants(True):
+        r = _gpu(kind, Ar, Ai, js)
+    rc0 = _gpu(kind, Ar, Ai, js)
+    with vjo.secants(True):
+        o1 = _oracle(kind, Ar, Ai, js, 1)
+        o0 = _oracle(kind, Ar, Ai, js, 0)
+    assert r["rc"] == o1["rc"] and r["rc"] > 0
+    for f in ("eS", "fS", "G", "V"):
+        assert np.array_equal(r[f], o1[f], equal_nan=True), f
+    sg = np.sort(S.sigma(r["eS"], r["fS"]))
+    so = np.sort(S.sigma(o0["eS"], o0["fS"]))
+    rel = float(np.max(np.abs(sg - so) / so))
+    print(f"secants {kind} {m}x{n}: sweeps {r['rc']} (mode 0 {o0['rc']}), max rel err vs mode 0 {rel / (8 * n * EPS):.3f} x 8n eps")
+    assert rel <= 8 * n * EPS and abs(r["rc"] - o0["rc"]) <= 1
+    assert not np.array_equal(r["V"], rc0["V"])

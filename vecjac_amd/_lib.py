@@ -59,6 +59,8 @@ def lib() -> C.CDLL:
     _decl(L, "pjs_free", None, [P(C.c_uint)])
     _decl(L, "vjh_last_error", C.c_char_p, [])
     _decl(L, "vjh_abi_version", C.c_int, [])
+    _decl(L, "vjh_set_secants", None, [C.c_int])
+    _decl(L, "vjh_get_secants", C.c_int, [])
     _declare_svd(L)
     _lib = L
     return L

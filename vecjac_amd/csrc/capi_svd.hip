@@ -71,6 +71,11 @@ __global__ void k_table_check(const unsigned *js, unsigned n, int *bad)
 // rotation phases: Tn..Te are written as 0 and Tr holds the sweeps' GPU time.
 std::atomic<int> g_jtrace{0};
 
+// The reference's compile-time USE_SECANTS form (vecdef.h:48-52,97-101,
+// dvjsvd.c:411-415), switched at run time with vjh_set_secants: the pair
+// decisions take secants from the 2x2 EVD and the rotations divide by them.
+std::atomic<int> g_secants{0};
+
 void write_trace(FILE *f, const PlainRun &r, int rc, bool input_error)
 {
   (void)fprintf(f, "M=");
@@ -151,6 +156,7 @@ int host_vjsvd(const vj_fnat *m, const vj_fnat *n, T *Gr, const vj_fnat *ldGr, T
   a.stp = *stp;
   a.swp = *swp;
   a.vpreset = (*iwork != 0u);
+  a.sec = g_secants.load();
   unsigned *djs = nullptr;
   if (int rc = sg.alloc(&a.Gr, M * N)) return rc;
   if (int rc = sg.alloc(&a.Vr, N * N)) return rc;
@@ -240,6 +246,7 @@ int dev_vjsvd(vj_fnat m, vj_fnat n, T *Gr, T *Gi, vj_fnat ldG, T *Vr, T *Vi, vj_
   a.stp = stp;
   a.swp = swp;
   a.vpreset = (vpreset != 0u);
+  a.sec = g_secants.load();
   PlainRun run{};
   const int rc = plain_svd<T, CPLX>(a, &run, s);
   if (stats && rc > VJ_EHIP) {
@@ -319,6 +326,7 @@ int noalloc_dvjsvd(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_f
   a.stp = stp;
   a.swp = swp;
   a.vpreset = (vpreset != 0u);
+  a.sec = g_secants.load();
   PlainWork<double> w{(PlainState<double> *)(base + NA_STATE_OFF), (PlainSlot *)(base + NA_SLOT_OFF)};
   PlainRun run{};
   const int rc = plain_svd<double, false>(a, &run, s, &w);
@@ -343,6 +351,10 @@ vj_fint vjh_dvjsvd_noalloc(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double 
 }
 
 void vjh_set_jtrace(int on) { g_jtrace.store(on ? 1 : 0); }
+
+void vjh_set_secants(int on) { g_secants.store(on ? 1 : 0); }
+
+int vjh_get_secants(void) { return g_secants.load(); }
 
 vj_fint dvjsvd_(const vj_fnat *m, const vj_fnat *n, double *G, const vj_fnat *ldG, double *V, const vj_fnat *ldV,
                 double *eS, double *fS, const unsigned *js, const unsigned *stp, const unsigned *swp, double *work,

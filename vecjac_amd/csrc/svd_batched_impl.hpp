@@ -1317,6 +1317,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       S.sG = 0;
       S.sV = 0;
       S.vtrack = 0;
+      S.sec = 0;
       int par = 0, gstep = 0;
       for (sw = 0; sw < a.swp;) {
         unsigned long long swt = 0;

@@ -25,6 +25,7 @@ template <typename T> struct PlainArgs {
   T tol;              // convergence threshold (0: sqrt(m) eps/2)
   int raw;            // no epilogue: G, V and eS stay in their scaled working form
   int stats;          // record max |cos| and max |sin| of the pairs of every sweep
+  int sec;            // the reference's USE_SECANTS form: secants, rotations divide by them
 };
 
 // The first two words are copied to the host once per sweep.

@@ -327,6 +327,7 @@ template <typename T> struct Step {
   T tol, gst;        // sqrt(m) eps/2 and tol * 2^FIN_EXP (dvjsvd.c:183-184)
   int sG, sV;        // pending power-of-two rescale of G and V in this step
   int vtrack;        // 1: V preset (djrot_ tracking), 0: V = I (djrotf_)
+  int sec;           // 1: the USE_SECANTS form (secants, rotations divide)
 };
 
 // Results of pair_reduce: norms of the pair (p = x, q = y) and the cosine
@@ -341,9 +342,18 @@ template <typename T> struct PairDec {
   int code;          // action code (dvjsvd.c:438-453): 3, -3, 2, -2, -1, 1
   int trans;         // 1 if transformed (counted in the sweep total)
   int err;           // 0, or the reference's negative line code
-  T c, at, sat;      // cosine and (complex) tangent of the rotation
+  T c, at, sat;      // cosine (secant when sec) and (complex) tangent of the rotation
   T cosv;            // |a21|: |cos| of the angle between the two columns
+  int sec;           // c is the secant: the rotations divide by it
 };
+
+// The scaling of a rotated element: the product by the cosine, or in the
+// reference's USE_SECANTS form (VDMULDIV, vecdef.h:97-101) the quotient by
+// the secant.
+template <typename T> __device__ __forceinline__ T cscale(const T v, const T c, const int sec)
+{
+  return sec ? (v / c) : (v * c);
+}
 
 template <typename T, bool CPLX> struct PairOut {
   int code;
@@ -475,11 +485,12 @@ pair_reduce(const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[
 }
 
 // Convergence test, Gram prescaling, 2x2 EVD and action code of one pair
-// (dvjsvd.c:349-459, zvjsvd.c:377-491); per lane.
+// (dvjsvd.c:349-459, zvjsvd.c:377-491); per lane.  sec: the 2x2 EVD returns
+// the secant in c (dbjac2i with n < 0, dvjsvd.c:411-415).
 template <typename T, bool CPLX>
-__device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T tol, const T gst)
+__device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T tol, const T gst, const int sec = 0)
 {
-  PairDec<T> d{1, 0, 0, (T)1, (T)0, (T)0, (T)0};
+  PairDec<T> d{1, 0, 0, (T)1, (T)0, (T)0, (T)0, sec};
   if (!isfinite_(P.a21r) || !isfinite_(P.a21i)) {
     d.err = CPLX ? -345 : -320;
     return d;
@@ -520,10 +531,18 @@ __device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T 
     const T w0 = scalef(f12, e12), w1 = scalef(f21, e21);
     T l1, l2;
     bool Pm;
-    if constexpr (CPLX)
-      cjac2<T, JAC2_TAN>(w0, w1, scalef(P.a21r, dd), scalef(P.a21i, dd), d.c, d.at, d.sat, l1, l2, Pm);
-    else
-      rjac2<T, JAC2_TAN>(w0, w1, scalef(P.a21r, dd), d.c, d.at, l1, l2, Pm);
+    if constexpr (CPLX) {
+      if (sec)
+        cjac2<T, JAC2_SEC>(w0, w1, scalef(P.a21r, dd), scalef(P.a21i, dd), d.c, d.at, d.sat, l1, l2, Pm);
+      else
+        cjac2<T, JAC2_TAN>(w0, w1, scalef(P.a21r, dd), scalef(P.a21i, dd), d.c, d.at, d.sat, l1, l2, Pm);
+    }
+    else {
+      if (sec)
+        rjac2<T, JAC2_SEC>(w0, w1, scalef(P.a21r, dd), d.c, d.at, l1, l2, Pm);
+      else
+        rjac2<T, JAC2_TAN>(w0, w1, scalef(P.a21r, dd), d.c, d.at, l1, l2, Pm);
+    }
     d.code = (!ngs && cgs) ? 3 : ((ngs && cgs) ? -3 : (Pm ? -2 : 2));
   }
   else
@@ -537,7 +556,8 @@ __device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T 
 // when PERM, the scaling omitted when C1 (c == 1).  Returns the lane's max
 // |updated element|.
 template <typename T, bool CPLX, int E, bool PERM, bool C1>
-__device__ __forceinline__ T rot_real(T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E], const T c, const T at, const int nE)
+__device__ __forceinline__ T rot_real(T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E], const T c, const T at, const int nE,
+                                      const int sec)
 {
   T tG = (T)0;
 #pragma unroll
@@ -546,14 +566,14 @@ __device__ __forceinline__ T rot_real(T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi
       break;
     const T u = fma_(yr[j], at, xr[j]), v = fma_(-xr[j], at, yr[j]);
     T a = PERM ? v : u, b = PERM ? u : v;
-    if constexpr (!C1) { a = a * c; b = b * c; }
+    if constexpr (!C1) { a = cscale(a, c, sec); b = cscale(b, c, sec); }
     xr[j] = a;
     yr[j] = b;
     tG = max1p(tG, max1p(vabs(a), vabs(b)));
     if constexpr (CPLX) {
       const T u2 = fma_(yi[j], at, xi[j]), v2 = fma_(-xi[j], at, yi[j]);
       T a2 = PERM ? v2 : u2, b2 = PERM ? u2 : v2;
-      if constexpr (!C1) { a2 = a2 * c; b2 = b2 * c; }
+      if constexpr (!C1) { a2 = cscale(a2, c, sec); b2 = cscale(b2, c, sec); }
       xi[j] = a2;
       yi[j] = b2;
       tG = max1p(tG, max1p(vabs(a2), vabs(b2)));
@@ -601,11 +621,11 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
       // the permutation and the unit cosine select the loop, not the element
       const bool c1 = (c == (T)1);
       if (perm)
-        tG = c1 ? rot_real<T, CPLX, E, true, true>(xr, yr, xi, yi, c, at, nE)
-                : rot_real<T, CPLX, E, true, false>(xr, yr, xi, yi, c, at, nE);
+        tG = c1 ? rot_real<T, CPLX, E, true, true>(xr, yr, xi, yi, c, at, nE, D.sec)
+                : rot_real<T, CPLX, E, true, false>(xr, yr, xi, yi, c, at, nE, D.sec);
       else
-        tG = c1 ? rot_real<T, CPLX, E, false, true>(xr, yr, xi, yi, c, at, nE)
-                : rot_real<T, CPLX, E, false, false>(xr, yr, xi, yi, c, at, nE);
+        tG = c1 ? rot_real<T, CPLX, E, false, true>(xr, yr, xi, yi, c, at, nE, D.sec)
+                : rot_real<T, CPLX, E, false, false>(xr, yr, xi, yi, c, at, nE, D.sec);
     }
     else {
       if constexpr (CPLX) {
@@ -620,7 +640,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
           T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
           T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
           T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
-          if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+          if (!c1) { ar = cscale(ar, c, D.sec); ai = cscale(ai, c, D.sec); br = cscale(br, c, D.sec); bi = cscale(bi, c, D.sec); }
           tG = xmax(tG, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
           if (perm) { xr[j] = br; xi[j] = bi; yr[j] = ar; yi[j] = ai; }
           else { xr[j] = ar; xi[j] = ai; yr[j] = br; yi[j] = bi; }
@@ -646,7 +666,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
         if (!CPLX || (sat == (T)0)) {
           T a = perm ? fma_(-x_r, at, y_r) : fma_(y_r, at, x_r);
           T b = perm ? fma_(y_r, at, x_r) : fma_(-x_r, at, y_r);
-          if (!c1) { a = a * c; b = b * c; }
+          if (!c1) { a = cscale(a, c, D.sec); b = cscale(b, c, D.sec); }
           Vrp[i] = a;
           Vrq[i] = b;
           tV = max1p(tV, max1p(vabs(a), vabs(b)));
@@ -654,7 +674,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
             const T x_i = Vip[i], y_i = Viq[i];
             T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
             T b2 = perm ? fma_(y_i, at, x_i) : fma_(-x_i, at, y_i);
-            if (!c1) { a2 = a2 * c; b2 = b2 * c; }
+            if (!c1) { a2 = cscale(a2, c, D.sec); b2 = cscale(b2, c, D.sec); }
             Vip[i] = a2;
             Viq[i] = b2;
             tV = max1p(tV, max1p(vabs(a2), vabs(b2)));
@@ -666,7 +686,7 @@ pair_apply(const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E],
           T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
           T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
           T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
-          if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+          if (!c1) { ar = cscale(ar, c, D.sec); ai = cscale(ai, c, D.sec); br = cscale(br, c, D.sec); bi = cscale(bi, c, D.sec); }
           tV = xmax(tV, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
           if (perm) { Vrp[i] = br; Vip[i] = bi; Vrq[i] = ar; Viq[i] = ai; }
           else { Vrp[i] = ar; Vip[i] = ai; Vrq[i] = br; Viq[i] = bi; }
@@ -798,7 +818,7 @@ __device__ __forceinline__ T v_apply(const PairDec<T> &D, T (&vx)[E], T (&vy)[E]
         if (!CPLX || (sat == (T)0)) {
           T a = perm ? fma_(-x_r, at, y_r) : fma_(y_r, at, x_r);
           T b = perm ? fma_(y_r, at, x_r) : fma_(-x_r, at, y_r);
-          if (!c1) { a = a * c; b = b * c; }
+          if (!c1) { a = cscale(a, c, D.sec); b = cscale(b, c, D.sec); }
           vx[j] = a;
           vy[j] = b;
           tV = max1p(tV, max1p(vabs(a), vabs(b)));
@@ -806,7 +826,7 @@ __device__ __forceinline__ T v_apply(const PairDec<T> &D, T (&vx)[E], T (&vy)[E]
             const T x_i = vxi[j], y_i = vyi[j];
             T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
             T b2 = perm ? fma_(y_i, at, x_i) : fma_(-x_i, at, y_i);
-            if (!c1) { a2 = a2 * c; b2 = b2 * c; }
+            if (!c1) { a2 = cscale(a2, c, D.sec); b2 = cscale(b2, c, D.sec); }
             vxi[j] = a2;
             vyi[j] = b2;
             tV = max1p(tV, max1p(vabs(a2), vabs(b2)));
@@ -818,7 +838,7 @@ __device__ __forceinline__ T v_apply(const PairDec<T> &D, T (&vx)[E], T (&vy)[E]
           T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
           T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
           T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
-          if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+          if (!c1) { ar = cscale(ar, c, D.sec); ai = cscale(ai, c, D.sec); br = cscale(br, c, D.sec); bi = cscale(bi, c, D.sec); }
           tV = xmax(tV, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
           if (perm) { vx[j] = br; vxi[j] = bi; vy[j] = ar; vyi[j] = ai; }
           else { vx[j] = ar; vxi[j] = ai; vy[j] = br; vyi[j] = bi; }
@@ -845,7 +865,7 @@ pair_step(const int m, const int n, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[
 {
   (void)m;
   const PairScal<T> P = pair_reduce<T, CPLX, B, E>(xr, yr, xi, yi, lds);
-  D = pair_decide<T, CPLX>(P, S.tol, S.gst);
+  D = pair_decide<T, CPLX>(P, S.tol, S.gst, S.sec);
   return pair_apply<T, CPLX, B, E>(n, xr, yr, xi, yi, Vrp, Vrq, Vip, Viq, P, D, S.vtrack, eSp, fSp, eSq, fSq, lds, lane);
 }
 

@@ -375,7 +375,8 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
                                              const long long ldV, T *__restrict__ eS, T *__restrict__ fS,
                                              const unsigned *__restrict__ row, const unsigned st_idx,
                                              const unsigned stp, const unsigned swp, const T tol, const T gst,
-                                             const int vtrack, const int stats, PlainState<T> *__restrict__ st,
+                                             const int vtrack, const int stats, const int sec,
+                                             PlainState<T> *__restrict__ st,
                                              const unsigned r, const PlainSlot *__restrict__ prev,
                                              PlainSlot *__restrict__ slots)
 {
@@ -432,6 +433,7 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   S.sG = c.sG;
   S.sV = c.sV;
   S.vtrack = vtrack;
+  S.sec = sec;
   if (!STREAM && S.sG)
     scale_rows<T, CPLX, E>(S.sG, xr, yr, xi, yi);
   if (vreg && S.sV)
@@ -541,7 +543,7 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
       red_dd4<A, B>(sx, sy, dr, di, lds);
     }
     const PairScal<T> P = pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
-    D = pair_decide<T, CPLX>(P, S.tol, S.gst);
+    D = pair_decide<T, CPLX>(P, S.tol, S.gst, S.sec);
     T tG = (T)0;
     for (int c = 0; c < nc; ++c) {
       load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
@@ -621,7 +623,7 @@ static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, unsigned r, co
 {
   hipLaunchKernelGGL((k_step<T, CPLX, B, E, STREAM>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m,
                      (int)(a.nv ? a.nv : a.n), a.Gr, a.Gi, a.ldG, a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp,
-                     tol, gst, a.vpreset, a.stats, st, r, prev, slots);
+                     tol, gst, a.vpreset, a.stats, a.sec, st, r, prev, slots);
   return 0;
 }
 

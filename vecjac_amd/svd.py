@@ -117,6 +117,25 @@ def device_vjsvd(kind: str, G, V, eS, fS, js_dev, stp: int, swp: int = 999, Gi=N
     return rc, (stats[0], stats[1], list(stats[2:2 + 64]))
 
 
+class secants:
+    """Context manager for the plain drivers' rotation form
+    (vjh_set_secants): on = the reference's USE_SECANTS build (the 2x2 EVD
+    returns secants and the rotations divide by them)."""
+
+    def __init__(self, on: bool = True):
+        self.on = int(bool(on))
+
+    def __enter__(self):
+        L = _lib.lib()
+        self.prev = L.vjh_get_secants()
+        L.vjh_set_secants(self.on)
+        return self
+
+    def __exit__(self, *exc):
+        _lib.lib().vjh_set_secants(self.prev)
+        return False
+
+
 def _i64(x):
     return C.byref(C.c_int64(int(x)))
 
