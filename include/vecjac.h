@@ -27,8 +27,22 @@
 
 #define VJH_ABI_VERSION 1
 
-typedef int64_t vj_fint;   /* fint: long long under MKL_ILP64 (common.h:184-186) */
-typedef uint64_t vj_fnat;  /* fnat: unsigned long long (common.h:185-186) */
+/* Integer types of the reference's own interfaces (fint, fnat; common.h:
+ * 180-190): 64-bit as under MKL_ILP64, the reference's build default, in
+ * libvecjac_hip.so; 32-bit (LP64: int, unsigned) in libvecjac_hip_lp64.so,
+ * the same sources compiled with -DVJ_LP64.  A caller compiles against this
+ * header with -DVJ_LP64 exactly when it links the LP64 library. */
+#ifdef VJ_LP64
+typedef int32_t vj_fint;   /* fint: int (common.h:187) */
+typedef uint32_t vj_fnat;  /* fnat: unsigned (common.h:188) */
+#else
+typedef int64_t vj_fint;   /* fint: long long under MKL_ILP64 (common.h:184) */
+typedef uint64_t vj_fnat;  /* fnat: unsigned long long (common.h:185) */
+#endif
+/* Integer types of this library's own entry points (vjh_*): 64-bit in both
+ * builds. */
+typedef int64_t vjh_int;
+typedef uint64_t vjh_nat;
 /* element type of interleaved complex arrays: double complex in C (as the
  * reference's prototypes), one (re, im) pair of doubles seen from C++ */
 #ifdef __cplusplus
@@ -171,21 +185,21 @@ void cnssvj_(const char *joba, const char *jobu, const char *jobv, const vj_fint
 
 /* Same computations as ?bjac2_ (and, vjh_?bjac2i, as ?bjac2i) on device
  * arrays, enqueued on `stream`; they return 0 on success. */
-vj_fint vjh_dbjac2(vj_fint n, const double *a11, const double *a22, const double *a21,
+vjh_int vjh_dbjac2(vjh_int n, const double *a11, const double *a22, const double *a21,
                    double *c, double *at, double *l1, double *l2, unsigned *p, void *stream);
-vj_fint vjh_zbjac2(vj_fint n, const double *a11, const double *a22, const double *a21r, const double *a21i,
+vjh_int vjh_zbjac2(vjh_int n, const double *a11, const double *a22, const double *a21r, const double *a21i,
                    double *c, double *cat, double *sat, double *l1, double *l2, unsigned *p, void *stream);
-vj_fint vjh_sbjac2(vj_fint n, const float *a11, const float *a22, const float *a21,
+vjh_int vjh_sbjac2(vjh_int n, const float *a11, const float *a22, const float *a21,
                    float *c, float *at, float *l1, float *l2, unsigned *p, void *stream);
-vj_fint vjh_cbjac2(vj_fint n, const float *a11, const float *a22, const float *a21r, const float *a21i,
+vjh_int vjh_cbjac2(vjh_int n, const float *a11, const float *a22, const float *a21r, const float *a21i,
                    float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream);
-vj_fint vjh_dbjac2i(vj_fint n, const double *a11, const double *a22, const double *a21,
+vjh_int vjh_dbjac2i(vjh_int n, const double *a11, const double *a22, const double *a21,
                     double *c, double *at, double *l1, double *l2, unsigned *p, void *stream);
-vj_fint vjh_zbjac2i(vj_fint n, const double *a11, const double *a22, const double *a21r, const double *a21i,
+vjh_int vjh_zbjac2i(vjh_int n, const double *a11, const double *a22, const double *a21r, const double *a21i,
                     double *c, double *cat, double *sat, double *l1, double *l2, unsigned *p, void *stream);
-vj_fint vjh_sbjac2i(vj_fint n, const float *a11, const float *a22, const float *a21,
+vjh_int vjh_sbjac2i(vjh_int n, const float *a11, const float *a22, const float *a21,
                     float *c, float *at, float *l1, float *l2, unsigned *p, void *stream);
-vj_fint vjh_cbjac2i(vj_fint n, const float *a11, const float *a22, const float *a21r, const float *a21i,
+vjh_int vjh_cbjac2i(vjh_int n, const float *a11, const float *a22, const float *a21r, const float *a21i,
                     float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream);
 
 /* Plain SVD on device memory: same driver as ?vjsvd_, with relaxed shape
@@ -195,14 +209,14 @@ vj_fint vjh_cbjac2i(vj_fint n, const float *a11, const float *a22, const float *
  * planes.  vpreset != 0 is the reference's iwork[0] != 0.  stats, if not
  * NULL, receives 66 words: sweeps, sweeps launched, and the number of
  * transformations in each of the first 64 sweeps.  Synchronises `stream`. */
-vj_fint vjh_dvjsvd(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS, double *fS,
+vjh_int vjh_dvjsvd(vjh_nat m, vjh_nat n, double *G, vjh_nat ldG, double *V, vjh_nat ldV, double *eS, double *fS,
                    const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats);
-vj_fint vjh_zvjsvd(vj_fnat m, vj_fnat n, double *Gr, double *Gi, vj_fnat ldG, double *Vr, double *Vi, vj_fnat ldV,
+vjh_int vjh_zvjsvd(vjh_nat m, vjh_nat n, double *Gr, double *Gi, vjh_nat ldG, double *Vr, double *Vi, vjh_nat ldV,
                    double *eS, double *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
                    void *stream, unsigned *stats);
-vj_fint vjh_svjsvd(vj_fnat m, vj_fnat n, float *G, vj_fnat ldG, float *V, vj_fnat ldV, float *eS, float *fS,
+vjh_int vjh_svjsvd(vjh_nat m, vjh_nat n, float *G, vjh_nat ldG, float *V, vjh_nat ldV, float *eS, float *fS,
                    const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats);
-vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, float *Vr, float *Vi, vj_fnat ldV,
+vjh_int vjh_cvjsvd(vjh_nat m, vjh_nat n, float *Gr, float *Gi, vjh_nat ldG, float *Vr, float *Vi, vjh_nat ldV,
                    float *eS, float *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
                    void *stream, unsigned *stats);
 
@@ -220,7 +234,7 @@ vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, floa
  * the convergence flag synchronously after each sweep (vjh_dvjsvd queues the
  * sweeps one ahead), so it may be marginally slower.  Synchronises `stream`. */
 #define VJH_NOALLOC_MAX_N 16384
-vj_fint vjh_dvjsvd_noalloc(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS,
+vjh_int vjh_dvjsvd_noalloc(vjh_nat m, vjh_nat n, double *G, vjh_nat ldG, double *V, vjh_nat ldV, double *eS,
                            double *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream,
                            unsigned *stats);
 
@@ -237,14 +251,14 @@ vj_fint vjh_dvjsvd_noalloc(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double 
  * and offS[b] of eS_b/fS_b in the packed device buffers.  waves: waves per
  * workgroup (1, 2, 4 or 8; 0 picks from the median width).  Returns NULL on
  * failure with *status < 0. */
-void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG,
-                       const vj_fint *offV, const vj_fint *offS, unsigned swp, int waves, vj_fint *status);
+void *vjh_bplan_create(int kind, vjh_int nb, const vjh_int *m, const vjh_int *n, const vjh_int *offG,
+                       const vjh_int *offV, const vjh_int *offS, unsigned swp, int waves, vjh_int *status);
 /* One run (the launches of the classes) on device buffers: G (A in, U out; U columns of zero singular
  * values are NaN as in the reference), V (out, starts as I), eS/fS (out),
  * info[b] (sweeps or a negative code), transforms[b] (rotations and
  * Gram-Schmidt steps of matrix b; may be NULL).  Gi/Vi are NULL for real
  * kinds.  Asynchronous on `stream`; graph-capturable. */
-vj_fint vjh_bplan_run(void *plan, void *G, void *Gi, void *V, void *Vi, void *eS, void *fS, vj_fint *info,
+vjh_int vjh_bplan_run(void *plan, void *G, void *Gi, void *V, void *Vi, void *eS, void *fS, vjh_int *info,
                       uint64_t *transforms, void *stream);
 /* Waves per workgroup of the first non-empty launch. */
 int vjh_bplan_waves(const void *plan);
@@ -253,7 +267,7 @@ int vjh_bplan_waves(const void *plan);
  * each launch, lanes per pair of each launch (64 for the wave form), the
  * split row count, the LDS budget and the dynamic LDS bytes per workgroup of
  * the LDS launch. */
-vj_fint vjh_bplan_config(const void *plan, vj_fint *out);
+vjh_int vjh_bplan_config(const void *plan, vjh_int *out);
 /* Sets the plan's choices: the split row count (at most 128, complex 64),
  * the LDS budget in bytes (at most 152 KiB; 0 keeps every matrix on global
  * memory) and, where not NULL, waves[3] per workgroup of the three launches
@@ -261,24 +275,24 @@ vj_fint vjh_bplan_config(const void *plan, vj_fint *out);
  * (lanes[1], lanes[2]: 16, 8 or 4; 8 lanes with at most 4 waves, 4 lanes
  * with at most 2 waves and real rows up to 64).  Returns -5 for a shape that
  * is not built.  The results do not depend on any of them. */
-vj_fint vjh_bplan_set(void *plan, int split_rows, vj_fint lds_budget, const int *waves, const int *lanes);
+vjh_int vjh_bplan_set(void *plan, int split_rows, vjh_int lds_budget, const int *waves, const int *lanes);
 /* Pair evaluations of each matrix in the plan's last run (uint64_t evals[nb]).
  * A pair whose columns did not change since its previous evaluation is not
  * evaluated again, so this can be below sweeps x steps x n/2; it is what the
  * run read.  Synchronises `stream`. */
-vj_fint vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream);
+vjh_int vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream);
 /* Elements of G and V (both planes of complex data) that each matrix's
  * solve in the plan's last run read from or wrote to device memory
  * (uint64_t elems[nb]): the column loads and stores of every step, the
  * set-up scan, V := I, the initial scaling and the epilogue; for a matrix
  * held in LDS, G read once and U and V written once.  Synchronises `stream`. */
-vj_fint vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream);
+vjh_int vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream);
 /* Ordering of the pairs in the plan's runs: 6 (default), the block modulus
  * ordering of pjs(6, n), whose long-column launch keeps the four columns of
  * a block pair in registers for the two steps of a round; or 4, the modulus
  * ordering of pjs(4, n).  A run equals the plain driver run with the same
  * table.  id 0 returns the current ordering; another id returns -2. */
-vj_fint vjh_bplan_order(void *plan, int id);
+vjh_int vjh_bplan_order(void *plan, int id);
 /* Autotuner of a plan, on copies of the input G (Gi) made into the scratch
  * buffer Gw (Gwi), each launch timed on its own with HIP events on `stream`:
  * (1) for the split row count and half of it, each launch at 1, 2, 4 and 8
@@ -291,19 +305,19 @@ vj_fint vjh_bplan_order(void *plan, int id);
  * four counts at the default split, -1 for skipped counts.  V, eS, fS and
  * info are overwritten.  Returns the count of the first non-empty launch,
  * or a negative code.  Synchronises `stream`. */
-vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
-                           void *fS, vj_fint *info, float *ms, void *stream);
+vjh_int vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
+                           void *fS, vjh_int *info, float *ms, void *stream);
 void vjh_bplan_destroy(void *plan);
 /* zsplit_ / zmerge_ on device memory (interleaved A as pairs of doubles),
  * one coalesced pass on `stream`, no host round trip. */
-vj_fint vjh_zsplit(vj_fnat m, vj_fnat n, const double *A, vj_fnat ldA, double *Ar, vj_fnat ldAr, double *Ai, vj_fnat ldAi,
+vjh_int vjh_zsplit(vjh_nat m, vjh_nat n, const double *A, vjh_nat ldA, double *Ar, vjh_nat ldAr, double *Ai, vjh_nat ldAi,
                    void *stream);
-vj_fint vjh_zmerge(vj_fnat m, vj_fnat n, const double *Ar, vj_fnat ldAr, const double *Ai, vj_fnat ldAi, double *A, vj_fnat ldA,
+vjh_int vjh_zmerge(vjh_nat m, vjh_nat n, const double *Ar, vjh_nat ldAr, const double *Ai, vjh_nat ldAi, double *A, vjh_nat ldA,
                    void *stream);
 
 /* One-shot FP64 batched SVD (plan, run, synchronise, destroy). */
-vj_fint vjh_dbvjsvd(vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG, const vj_fint *offV,
-                    const vj_fint *offS, double *G, double *V, double *eS, double *fS, vj_fint *info,
+vjh_int vjh_dbvjsvd(vjh_int nb, const vjh_int *m, const vjh_int *n, const vjh_int *offG, const vjh_int *offV,
+                    const vjh_int *offS, double *G, double *V, double *eS, double *fS, vjh_int *info,
                     uint64_t *transforms, unsigned swp, int waves, void *stream);
 
 /* Adjoint of the thin SVD A = U diag(s) V^T for nb matrices of m x n (m >= n,
@@ -313,14 +327,17 @@ vj_fint vjh_dbvjsvd(vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fin
  *        + (I - U U^T) dU S^-1 V^T,   F_ij = 1 / (s_j^2 - s_i^2), F_ii = 0.
  * Matrix b: U, dU, dA at b*m*n (ld m); V, dV at b*n*n (ld n); s, ds at b*n.
  * Asynchronous on `stream`. */
-vj_fint vjh_dsvd_bwd(vj_fint nb, vj_fint m, vj_fint n, const double *U, const double *s, const double *V,
+vjh_int vjh_dsvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const double *U, const double *s, const double *V,
                      const double *dU, const double *ds, const double *dV, double *dA, void *stream);
-vj_fint vjh_ssvd_bwd(vj_fint nb, vj_fint m, vj_fint n, const float *U, const float *s, const float *V,
+vjh_int vjh_ssvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const float *U, const float *s, const float *V,
                      const float *dU, const float *ds, const float *dV, float *dA, void *stream);
 
 /* Message of the last failure on the calling thread. */
 const char *vjh_last_error(void);
 int vjh_abi_version(void);
+/* Bits of vj_fint in the loaded library: 64 (libvecjac_hip.so) or 32
+ * (libvecjac_hip_lp64.so). */
+int vjh_fint_bits(void);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,10 @@
 """CPU checks of the C ABI and of the strategy tables (no GPU needed).
 
-* libvecjac_hip.so loads and exports every function include/vecjac.h
-  declares; liboracle_vj.so exports every function oracle/vjo.h declares.
+* libvecjac_hip.so and its LP64 twin libvecjac_hip_lp64.so load and export
+  every function include/vecjac.h declares, with 64- and 32-bit reference
+  integers respectively (vjh_fint_bits); the LP64 twin's host zsplit_ /
+  zmerge_ take 32-bit arguments; liboracle_vj.so exports every function
+  oracle/vjo.h declares.
 * pjs(): every row is a permutation of 0..n-1 (disjoint pairs, race-free
   column updates); id 2 (ME) covers every unordered pair exactly once per
   sweep in n-1 steps; id 4 (MM) covers every pair at least once in n steps;
@@ -31,6 +34,41 @@ def test_hip_library_exports_header():
     assert "dvjsvd_" in names and "dbjac2_" in names and "pjs" in names
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_lp64_library_exports_header():
+    import ctypes as C
+    from vecjac_amd import _lib
+    lp = _lib.LIB_PATH.with_name("libvecjac_hip_lp64.so")
+    L = C.CDLL(str(lp))
+    names = _declared(ROOT / "include" / "vecjac.h")
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.vjh_fint_bits() == 32
+    assert C.CDLL(str(_lib.LIB_PATH)).vjh_fint_bits() == 64
+
+
+def test_lp64_host_zsplit_zmerge():
+    """zsplit_/zmerge_ of the LP64 library read 32-bit m, n and leading
+    dimensions (zsplit.c:3, zmerge.c:3 under LP64)."""
+    import ctypes as C
+    from vecjac_amd import _lib
+    L = C.CDLL(str(_lib.LIB_PATH.with_name("libvecjac_hip_lp64.so")))
+    for nm in ("zsplit_", "zmerge_"):
+        getattr(L, nm).restype = C.c_int32
+        getattr(L, nm).argtypes = [C.c_void_p] * 8
+    U = lambda v: C.byref(C.c_uint32(v))  # noqa: E731
+    g = np.random.default_rng(3)
+    m, n, ld, ldr = 37, 11, 40, 39
+    A = (g.standard_normal((n, ld)) + 1j * g.standard_normal((n, ld))).astype(np.complex128)
+    Ar = np.zeros((n, ldr))
+    Ai = np.zeros((n, ldr))
+    assert L.zsplit_(U(m), U(n), A.ctypes.data, U(ld), Ar.ctypes.data, U(ldr), Ai.ctypes.data, U(ldr)) == 0
+    assert np.array_equal(Ar[:, :m], A.real[:, :m]) and np.array_equal(Ai[:, :m], A.imag[:, :m])
+    B = np.zeros((n, ld), dtype=np.complex128)
+    assert L.zmerge_(U(m), U(n), Ar.ctypes.data, U(ldr), Ai.ctypes.data, U(ldr), B.ctypes.data, U(ld)) == 0
+    assert np.array_equal(B[:, :m], A[:, :m])
+    assert L.zsplit_(U(m), U(n), A.ctypes.data, U(m - 1), Ar.ctypes.data, U(ldr), Ai.ctypes.data, U(ldr)) == -4
 
 
 def test_oracle_library_exports_header():

@@ -3,7 +3,11 @@
     python -m vecjac_amd.build          # HIP library + oracle (+ reference build)
 
 The HIP library is compiled with hipcc straight from vecjac_amd/csrc into
-the package directory, so the .so travels with the source tree.  Floating
+the package directory, so the .so travels with the source tree.  Two
+libraries are linked from it: libvecjac_hip.so, whose reference interfaces
+take 64-bit Fortran integers (the reference's MKL_ILP64 default), and
+libvecjac_hip_lp64.so, the C-ABI sources compiled again with -DVJ_LP64
+(32-bit integers, the reference's LP64 build) around the same kernels.  Floating
 point contraction is off (every fused multiply-add in the kernels is written
 out) and packed FP32 instructions are disabled.
 """
@@ -21,6 +25,9 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libvecjac_hip.so"
+LIB_LP64 = PKG / "libvecjac_hip_lp64.so"
+# the sources whose entry points take the reference's integer types
+ABI_SRCS = ("capi_batched.hip", "capi_bjac2.cpp", "capi_nssvj.hip", "capi_svd.hip", "svd_bwd.hip", "zsplit.hip")
 DRIVERS = PKG / "drivers"
 BIN = PKG / "bin"
 ARCH = os.environ.get("VJ_OFFLOAD_ARCH", "gfx950")
@@ -35,9 +42,9 @@ COMMON = [
 ]
 
 
-def _compile(src: Path) -> Path:
-    obj = BUILD / (src.name + ".o")
-    cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
+def _compile(src: Path, lp64: bool = False) -> Path:
+    obj = BUILD / (("lp64_" if lp64 else "") + src.name + ".o")
+    cmd = [HIPCC, *COMMON, *(["-DVJ_LP64"] if lp64 else []), "-c", str(src), "-o", str(obj)]
     if src.suffix == ".hip":
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -46,19 +53,26 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build_hip(verbose: bool = True) -> Path:
-    BUILD.mkdir(exist_ok=True)
-    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB.with_suffix(".so.tmp")
+def _link(objs, lib: Path) -> None:
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+
+
+def build_hip(verbose: bool = True) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
+    jobs = [(s, False) for s in srcs] + [(CSRC / n, True) for n in ABI_SRCS]
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        objs = list(ex.map(lambda j: _compile(*j), jobs))
+    _link(objs[:len(srcs)], LIB)
+    lp = {o.name[len("lp64_"):]: o for o in objs[len(srcs):]}
+    _link([lp.get(o.name, o) for o in objs[:len(srcs)]], LIB_LP64)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {LIB_LP64}")
     return LIB
 
 

@@ -122,7 +122,7 @@ int cap_waves(int kind, int max_m, int waves)
 // ev, if not null, holds NCLASS - 1 events recorded between the launches
 // (autotuner timing)
 template <typename T, bool CPLX>
-int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vj_fint *info,
+int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vjh_int *info,
              uint64_t *transforms, hipStream_t s, hipEvent_t *ev = nullptr, unsigned mask = 7u)
 {
   BatchArgs<T> a{};
@@ -168,7 +168,7 @@ int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, v
   return 0;
 }
 
-int run_any(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vj_fint *info,
+int run_any(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vjh_int *info,
             uint64_t *transforms, hipStream_t s, hipEvent_t *ev, unsigned mask = 7u)
 {
   switch (P->kind) {
@@ -206,7 +206,7 @@ std::vector<Shape> shape_candidates(const BPlan *P, int c)
 // the summed time of the launches at candidate k (k < 4: the waves 1, 2, 4,
 // 8 at 16 lanes per pair).
 int tune_shapes(BPlan *P, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS, void *fS,
-                vj_fint *info, unsigned mask, hipStream_t s, float *best, float *tot)
+                vjh_int *info, unsigned mask, hipStream_t s, float *best, float *tot)
 {
   const bool cplx = (P->kind & 1) != 0;
   const size_t bytes = (size_t)P->totG * (size_t)tsize(P->kind);
@@ -282,11 +282,11 @@ int tune_shapes(BPlan *P, const void *G, const void *Gi, void *Gw, void *Gwi, vo
 
 extern "C" {
 
-void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG,
-                       const vj_fint *offV, const vj_fint *offS, unsigned swp, int waves, vj_fint *status)
+void *vjh_bplan_create(int kind, vjh_int nb, const vjh_int *m, const vjh_int *n, const vjh_int *offG,
+                       const vjh_int *offV, const vjh_int *offS, unsigned swp, int waves, vjh_int *status)
 {
-  vj_fint dummy = 0;
-  vj_fint *st = status ? status : &dummy;
+  vjh_int dummy = 0;
+  vjh_int *st = status ? status : &dummy;
   *st = 0;
   if (kind < 0 || kind > 3 || nb < 0) {
     *st = set_error(-1, "bad kind or batch size");
@@ -294,7 +294,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   }
   std::vector<long long> hm(m, m + nb), hn(n, n + nb);
   int max_m = 0;
-  for (vj_fint b = 0; b < nb; ++b) {
+  for (vjh_int b = 0; b < nb; ++b) {
     if (hn[b] < 2 || (hn[b] & 1) || hn[b] > 128 || hm[b] < hn[b] || hm[b] > 1024 || offG[b] < 0 || offV[b] < 0 ||
         offS[b] < 0) {
       *st = set_error(-2, "matrix %lld: need even 2 <= n <= 128, n <= m <= 1024 and non-negative offsets (m=%lld n=%lld)",
@@ -304,7 +304,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
     max_m = std::max(max_m, (int)hm[b]);
   }
   long long totG = 0;
-  for (vj_fint b = 0; b < nb; ++b)
+  for (vjh_int b = 0; b < nb; ++b)
     totG = std::max(totG, (long long)offG[b] + hm[b] * hn[b]);
   BPlan *P = new BPlan();
   P->kind = kind;
@@ -348,7 +348,7 @@ void *vjh_bplan_create(int kind, vj_fint nb, const vj_fint *m, const vj_fint *n,
   return P;
 }
 
-vj_fint vjh_bplan_run(void *plan, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vj_fint *info,
+vjh_int vjh_bplan_run(void *plan, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, void *fS, vjh_int *info,
                       uint64_t *transforms, void *stream)
 {
   const BPlan *P = (const BPlan *)plan;
@@ -368,7 +368,7 @@ int vjh_bplan_waves(const void *plan)
   return P->waves_c[0];
 }
 
-vj_fint vjh_bplan_config(const void *plan, vj_fint *out)
+vjh_int vjh_bplan_config(const void *plan, vjh_int *out)
 {
   const BPlan *P = (const BPlan *)plan;
   if (!P || !out)
@@ -385,7 +385,7 @@ vj_fint vjh_bplan_config(const void *plan, vj_fint *out)
   return 0;
 }
 
-vj_fint vjh_bplan_set(void *plan, int split_rows, vj_fint lds_budget, const int *waves, const int *lanes)
+vjh_int vjh_bplan_set(void *plan, int split_rows, vjh_int lds_budget, const int *waves, const int *lanes)
 {
   BPlan *P = (BPlan *)plan;
   if (!P)
@@ -416,7 +416,7 @@ vj_fint vjh_bplan_set(void *plan, int split_rows, vj_fint lds_budget, const int 
   return 0;
 }
 
-vj_fint vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream)
+vjh_int vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream)
 {
   const BPlan *P = (const BPlan *)plan;
   if (!P || !evals)
@@ -428,7 +428,7 @@ vj_fint vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream)
   return 0;
 }
 
-vj_fint vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream)
+vjh_int vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream)
 {
   const BPlan *P = (const BPlan *)plan;
   if (!P || !elems)
@@ -440,7 +440,7 @@ vj_fint vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream)
   return 0;
 }
 
-vj_fint vjh_bplan_order(void *plan, int id)
+vjh_int vjh_bplan_order(void *plan, int id)
 {
   BPlan *P = (BPlan *)plan;
   if (!P)
@@ -453,8 +453,8 @@ vj_fint vjh_bplan_order(void *plan, int id)
   return id;
 }
 
-vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
-                           void *fS, vj_fint *info, float *ms, void *stream)
+vjh_int vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS,
+                           void *fS, vjh_int *info, float *ms, void *stream)
 {
   BPlan *P = (BPlan *)plan;
   if (!P)
@@ -525,7 +525,7 @@ vj_fint vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, 
   P->waves_c[2] = bw2;
   P->lg_c[1] = bl1;
   P->lg_c[2] = bl2;
-  return (vj_fint)vjh_bplan_waves(P);
+  return (vjh_int)vjh_bplan_waves(P);
 }
 
 void vjh_bplan_destroy(void *plan)
@@ -541,15 +541,15 @@ void vjh_bplan_destroy(void *plan)
   delete P;
 }
 
-vj_fint vjh_dbvjsvd(vj_fint nb, const vj_fint *m, const vj_fint *n, const vj_fint *offG, const vj_fint *offV,
-                    const vj_fint *offS, double *G, double *V, double *eS, double *fS, vj_fint *info,
+vjh_int vjh_dbvjsvd(vjh_int nb, const vjh_int *m, const vjh_int *n, const vjh_int *offG, const vjh_int *offV,
+                    const vjh_int *offS, double *G, double *V, double *eS, double *fS, vjh_int *info,
                     uint64_t *transforms, unsigned swp, int waves, void *stream)
 {
-  vj_fint st = 0;
+  vjh_int st = 0;
   void *P = vjh_bplan_create(0, nb, m, n, offG, offV, offS, swp, waves, &st);
   if (!P)
     return st;
-  const vj_fint rc = vjh_bplan_run(P, G, nullptr, V, nullptr, eS, fS, info, transforms, stream);
+  const vjh_int rc = vjh_bplan_run(P, G, nullptr, V, nullptr, eS, fS, info, transforms, stream);
   (void)hipStreamSynchronize((hipStream_t)stream);
   vjh_bplan_destroy(P);
   return rc;

@@ -144,41 +144,42 @@ vj_fint cbjac2i(const vj_fint *n, const float *a11, const float *a22, const floa
   return host_cbjac2<float, true>(n, a11, a22, a21r, a21i, c, cat, sat, l1, l2, p);
 }
 
-vj_fint vjh_dbjac2(vj_fint n, const double *a11, const double *a22, const double *a21, double *c, double *at, double *l1, double *l2, unsigned *p, void *stream)
+vjh_int vjh_dbjac2(vjh_int n, const double *a11, const double *a22, const double *a21, double *c, double *at, double *l1, double *l2, unsigned *p, void *stream)
 {
   return launch_rbjac2<double>((long long)n, a11, a22, a21, c, at, l1, l2, p, (hipStream_t)stream);
 }
-vj_fint vjh_sbjac2(vj_fint n, const float *a11, const float *a22, const float *a21, float *c, float *at, float *l1, float *l2, unsigned *p, void *stream)
+vjh_int vjh_sbjac2(vjh_int n, const float *a11, const float *a22, const float *a21, float *c, float *at, float *l1, float *l2, unsigned *p, void *stream)
 {
   return launch_rbjac2<float>((long long)n, a11, a22, a21, c, at, l1, l2, p, (hipStream_t)stream);
 }
-vj_fint vjh_zbjac2(vj_fint n, const double *a11, const double *a22, const double *a21r, const double *a21i, double *c, double *cat, double *sat, double *l1, double *l2, unsigned *p, void *stream)
+vjh_int vjh_zbjac2(vjh_int n, const double *a11, const double *a22, const double *a21r, const double *a21i, double *c, double *cat, double *sat, double *l1, double *l2, unsigned *p, void *stream)
 {
   return launch_cbjac2<double>((long long)n, a11, a22, a21r, a21i, c, cat, sat, l1, l2, p, (hipStream_t)stream);
 }
-vj_fint vjh_cbjac2(vj_fint n, const float *a11, const float *a22, const float *a21r, const float *a21i, float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream)
+vjh_int vjh_cbjac2(vjh_int n, const float *a11, const float *a22, const float *a21r, const float *a21i, float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream)
 {
   return launch_cbjac2<float>((long long)n, a11, a22, a21r, a21i, c, cat, sat, l1, l2, p, (hipStream_t)stream);
 }
 
-vj_fint vjh_dbjac2i(vj_fint n, const double *a11, const double *a22, const double *a21, double *c, double *at, double *l1, double *l2, unsigned *p, void *stream)
+vjh_int vjh_dbjac2i(vjh_int n, const double *a11, const double *a22, const double *a21, double *c, double *at, double *l1, double *l2, unsigned *p, void *stream)
 {
   return launch_rbjac2i<double>((long long)n, a11, a22, a21, c, at, l1, l2, p, (hipStream_t)stream);
 }
-vj_fint vjh_sbjac2i(vj_fint n, const float *a11, const float *a22, const float *a21, float *c, float *at, float *l1, float *l2, unsigned *p, void *stream)
+vjh_int vjh_sbjac2i(vjh_int n, const float *a11, const float *a22, const float *a21, float *c, float *at, float *l1, float *l2, unsigned *p, void *stream)
 {
   return launch_rbjac2i<float>((long long)n, a11, a22, a21, c, at, l1, l2, p, (hipStream_t)stream);
 }
-vj_fint vjh_zbjac2i(vj_fint n, const double *a11, const double *a22, const double *a21r, const double *a21i, double *c, double *cat, double *sat, double *l1, double *l2, unsigned *p, void *stream)
+vjh_int vjh_zbjac2i(vjh_int n, const double *a11, const double *a22, const double *a21r, const double *a21i, double *c, double *cat, double *sat, double *l1, double *l2, unsigned *p, void *stream)
 {
   return launch_cbjac2i<double>((long long)n, a11, a22, a21r, a21i, c, cat, sat, l1, l2, p, (hipStream_t)stream);
 }
-vj_fint vjh_cbjac2i(vj_fint n, const float *a11, const float *a22, const float *a21r, const float *a21i, float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream)
+vjh_int vjh_cbjac2i(vjh_int n, const float *a11, const float *a22, const float *a21r, const float *a21i, float *c, float *cat, float *sat, float *l1, float *l2, unsigned *p, void *stream)
 {
   return launch_cbjac2i<float>((long long)n, a11, a22, a21r, a21i, c, cat, sat, l1, l2, p, (hipStream_t)stream);
 }
 
 const char *vjh_last_error(void) { return vj::last_error(); }
 int vjh_abi_version(void) { return VJH_ABI_VERSION; }
+int vjh_fint_bits(void) { return (int)(8 * sizeof(vj_fint)); }
 
 } // extern "C"

@@ -209,7 +209,7 @@ int host_vjsvd(const vj_fnat *m, const vj_fnat *n, T *Gr, const vj_fnat *ldGr, T
 }
 
 template <typename T, bool CPLX>
-int dev_vjsvd(vj_fnat m, vj_fnat n, T *Gr, T *Gi, vj_fnat ldG, T *Vr, T *Vi, vj_fnat ldV, T *eS, T *fS,
+int dev_vjsvd(vjh_nat m, vjh_nat n, T *Gr, T *Gi, vjh_nat ldG, T *Vr, T *Vi, vjh_nat ldV, T *eS, T *fS,
               const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats)
 {
   if (!n)
@@ -276,14 +276,14 @@ constexpr int NA_MAX_DEV = 64;
 std::mutex g_na_mu[NA_MAX_DEV];            // one call at a time per GPU
 unsigned char *g_na_ptr[NA_MAX_DEV] = {};  // device address of each GPU's scratch
 
-int noalloc_dvjsvd(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS, double *fS,
+int noalloc_dvjsvd(vjh_nat m, vjh_nat n, double *G, vjh_nat ldG, double *V, vjh_nat ldV, double *eS, double *fS,
                    const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats)
 {
   if (!n)
     return 0;
   if (m < n) return -1;
   if (n & 1u) return -2;
-  if (n > (vj_fnat)VJH_NOALLOC_MAX_N)
+  if (n > (vjh_nat)VJH_NOALLOC_MAX_N)
     return set_error(-2, "n = %llu exceeds the %d columns the allocation-free scratch holds",
                      (unsigned long long)n, VJH_NOALLOC_MAX_N);
   if (ldG < m) return -4;
@@ -343,7 +343,7 @@ int noalloc_dvjsvd(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_f
 
 extern "C" {
 
-vj_fint vjh_dvjsvd_noalloc(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS,
+vjh_int vjh_dvjsvd_noalloc(vjh_nat m, vjh_nat n, double *G, vjh_nat ldG, double *V, vjh_nat ldV, double *eS,
                            double *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream,
                            unsigned *stats)
 {
@@ -381,23 +381,23 @@ vj_fint cvjsvd_(const vj_fnat *m, const vj_fnat *n, float *Gr, const vj_fnat *ld
   return host_vjsvd<float, true>(m, n, Gr, ldGr, Gi, ldGi, Vr, ldVr, Vi, ldVi, eS, fS, js, stp, swp, work, iwork);
 }
 
-vj_fint vjh_dvjsvd(vj_fnat m, vj_fnat n, double *G, vj_fnat ldG, double *V, vj_fnat ldV, double *eS, double *fS,
+vjh_int vjh_dvjsvd(vjh_nat m, vjh_nat n, double *G, vjh_nat ldG, double *V, vjh_nat ldV, double *eS, double *fS,
                    const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats)
 {
   return dev_vjsvd<double, false>(m, n, G, nullptr, ldG, V, nullptr, ldV, eS, fS, js, stp, swp, vpreset, stream, stats);
 }
-vj_fint vjh_svjsvd(vj_fnat m, vj_fnat n, float *G, vj_fnat ldG, float *V, vj_fnat ldV, float *eS, float *fS,
+vjh_int vjh_svjsvd(vjh_nat m, vjh_nat n, float *G, vjh_nat ldG, float *V, vjh_nat ldV, float *eS, float *fS,
                    const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats)
 {
   return dev_vjsvd<float, false>(m, n, G, nullptr, ldG, V, nullptr, ldV, eS, fS, js, stp, swp, vpreset, stream, stats);
 }
-vj_fint vjh_zvjsvd(vj_fnat m, vj_fnat n, double *Gr, double *Gi, vj_fnat ldG, double *Vr, double *Vi, vj_fnat ldV,
+vjh_int vjh_zvjsvd(vjh_nat m, vjh_nat n, double *Gr, double *Gi, vjh_nat ldG, double *Vr, double *Vi, vjh_nat ldV,
                    double *eS, double *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
                    void *stream, unsigned *stats)
 {
   return dev_vjsvd<double, true>(m, n, Gr, Gi, ldG, Vr, Vi, ldV, eS, fS, js, stp, swp, vpreset, stream, stats);
 }
-vj_fint vjh_cvjsvd(vj_fnat m, vj_fnat n, float *Gr, float *Gi, vj_fnat ldG, float *Vr, float *Vi, vj_fnat ldV,
+vjh_int vjh_cvjsvd(vjh_nat m, vjh_nat n, float *Gr, float *Gi, vjh_nat ldG, float *Vr, float *Vi, vjh_nat ldV,
                    float *eS, float *fS, const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset,
                    void *stream, unsigned *stats)
 {

@@ -255,12 +255,12 @@ template int svd_bwd<float>(int, int, int, const float *, const float *, const f
 extern "C" {
 
 // dA for a batch of nb thin SVDs of m x n matrices; see the header.
-vj_fint vjh_dsvd_bwd(vj_fint nb, vj_fint m, vj_fint n, const double *U, const double *s, const double *V,
+vjh_int vjh_dsvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const double *U, const double *s, const double *V,
                      const double *dU, const double *ds, const double *dV, double *dA, void *stream)
 {
   return vj::svd_bwd<double>((int)nb, (int)m, (int)n, U, s, V, dU, ds, dV, dA, (hipStream_t)stream);
 }
-vj_fint vjh_ssvd_bwd(vj_fint nb, vj_fint m, vj_fint n, const float *U, const float *s, const float *V,
+vjh_int vjh_ssvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const float *U, const float *s, const float *V,
                      const float *dU, const float *ds, const float *dV, float *dA, void *stream)
 {
   return vj::svd_bwd<float>((int)nb, (int)m, (int)n, U, s, V, dU, ds, dV, dA, (hipStream_t)stream);

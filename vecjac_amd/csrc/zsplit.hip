@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_zmerge(const long long m, const long lo
 
 // argument checks of the reference (zsplit.c:6-23, debug builds): m, the
 // leading dimensions; alignment is not required here
-int check(vj_fnat m, vj_fnat ldA, vj_fnat ldAr, vj_fnat ldAi)
+int check(vjh_nat m, vjh_nat ldA, vjh_nat ldAr, vjh_nat ldAi)
 {
   if (ldA < m)
     return -4;
@@ -50,7 +50,7 @@ int check(vj_fnat m, vj_fnat ldA, vj_fnat ldAr, vj_fnat ldAi)
   return 0;
 }
 
-int launch_dims(vj_fnat m, vj_fnat n, dim3 &g)
+int launch_dims(vjh_nat m, vjh_nat n, dim3 &g)
 {
   if (n > 65535u)
     return set_error(-2, "n = %llu exceeds the grid limit of the split kernels", (unsigned long long)n);
@@ -94,7 +94,7 @@ vj_fint zmerge_(const vj_fnat *m, const vj_fnat *n, const double *Ar, const vj_f
   return 0;
 }
 
-vj_fint vjh_zsplit(vj_fnat m, vj_fnat n, const double *A, vj_fnat ldA, double *Ar, vj_fnat ldAr, double *Ai, vj_fnat ldAi,
+vjh_int vjh_zsplit(vjh_nat m, vjh_nat n, const double *A, vjh_nat ldA, double *Ar, vjh_nat ldAr, double *Ai, vjh_nat ldAi,
                    void *stream)
 {
   if (int rc = check(m, ldA, ldAr, ldAi))
@@ -109,7 +109,7 @@ vj_fint vjh_zsplit(vj_fnat m, vj_fnat n, const double *A, vj_fnat ldA, double *A
   return check_launch();
 }
 
-vj_fint vjh_zmerge(vj_fnat m, vj_fnat n, const double *Ar, vj_fnat ldAr, const double *Ai, vj_fnat ldAi, double *A, vj_fnat ldA,
+vjh_int vjh_zmerge(vjh_nat m, vjh_nat n, const double *Ar, vjh_nat ldAr, const double *Ai, vjh_nat ldAi, double *A, vjh_nat ldA,
                    void *stream)
 {
   if (ldAr < m)
