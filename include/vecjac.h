@@ -320,17 +320,32 @@ vjh_int vjh_dbvjsvd(vjh_int nb, const vjh_int *m, const vjh_int *n, const vjh_in
                     const vjh_int *offS, double *G, double *V, double *eS, double *fS, vjh_int *info,
                     uint64_t *transforms, unsigned swp, int waves, void *stream);
 
-/* Adjoint of the thin SVD A = U diag(s) V^T for nb matrices of m x n (m >= n,
- * n <= 64, (2mn + 3n^2 + 2n) elements <= 64 KiB): given U (m x n), s, V
- * (n x n) and the cotangents dU, ds, dV, writes
+/* Adjoint of the thin SVD A = U diag(s) V^T for nb matrices of m x n (any
+ * 1 <= n <= m; distinct nonzero s): given U (m x n), s, V (n x n) and the
+ * cotangents dU, ds, dV, writes
  *   dA = U [diag(ds) + (F o (U^T dU - dU^T U)) S + S (F o (V^T dV - dV^T V))] V^T
  *        + (I - U U^T) dU S^-1 V^T,   F_ij = 1 / (s_j^2 - s_i^2), F_ii = 0.
  * Matrix b: U, dU, dA at b*m*n (ld m); V, dV at b*n*n (ld n); s, ds at b*n.
+ * The n x n intermediates stay in LDS up to n = 96 (FP64; complex 64);
+ * wider matrices take a stream-ordered workspace (hipMallocAsync).
  * Asynchronous on `stream`. */
 vjh_int vjh_dsvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const double *U, const double *s, const double *V,
                      const double *dU, const double *ds, const double *dV, double *dA, void *stream);
 vjh_int vjh_ssvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const float *U, const float *s, const float *V,
                      const float *dU, const float *ds, const float *dV, float *dA, void *stream);
+/* Complex form (split real and imaginary planes, A = U diag(s) V^H), for a
+ * loss that does not depend on the phases of the singular vector pairs
+ * (u_k, v_k) -> (u_k e^{i phi}, v_k e^{i phi}): with J = U^H dU, K = V^H dV,
+ *   dA = U [diag(ds) + (F o (J - J^H)) S + S (F o (K - K^H))
+ *           + i diag(Im J_kk - Im K_kk) / (2 S)] V^H + (I - U U^H) dU S^-1 V^H.
+ * The cotangents follow the convention dL = Re sum(conj(dX) o dX') (torch's
+ * complex gradients). */
+vjh_int vjh_zsvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const double *Ur, const double *Ui, const double *s,
+                     const double *Vr, const double *Vi, const double *dUr, const double *dUi, const double *ds,
+                     const double *dVr, const double *dVi, double *dAr, double *dAi, void *stream);
+vjh_int vjh_csvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const float *Ur, const float *Ui, const float *s,
+                     const float *Vr, const float *Vi, const float *dUr, const float *dUi, const float *ds,
+                     const float *dVr, const float *dVi, float *dAr, float *dAi, void *stream);
 
 /* Message of the last failure on the calling thread. */
 const char *vjh_last_error(void);

@@ -1,5 +1,5 @@
 // Adjoint (backward) of the thin real SVD A = U diag(s) V^T for a batch of
-// matrices (no reference counterpart).  With F_ij = 1/(s_j^2 - s_i^2) for
+// matrices (no reference counterpart; the complex form is k_csvd_bwd below).  With F_ij = 1/(s_j^2 - s_i^2) for
 // i != j (0 on the diagonal), J = U^T dU and K = V^T dV:
 //   dA = U [diag(ds) + (F o (J - J^T)) S + S (F o (K - K^T))] V^T
 //        + (I - U U^T) dU S^-1 V^T
@@ -64,28 +64,113 @@ constexpr int TILE_LD = 17; // per-wave 16 x 16 transpose tile, padded
 
 } // namespace
 
-// LDS: J (then N) and K (then P), n16 x (n16 + 1) each; 1/s (n16); one
-// 16 x 17 tile per wave.
-template <typename T> static size_t bwd_lds_bytes(int n16)
+// LDS: J (then N) and K (then P), n16 x (n16 + 1) each (in a global
+// workspace instead when GW); 1/s (n16); one 16 x 17 tile per wave.  PL
+// planes: 1 real, 2 complex (real and imaginary parts).
+template <typename T> static size_t bwd_jk_elems(int n16, int pl) { return (size_t)2 * pl * n16 * (n16 + 1); }
+template <typename T> static size_t bwd_lds_bytes(int n16, int pl, bool gw)
 {
-  return ((size_t)2 * n16 * (n16 + 1) + n16 + (size_t)BWD_WAVES * 16 * TILE_LD) * sizeof(T);
+  return ((gw ? 0 : bwd_jk_elems<T>(n16, pl)) + n16 + (size_t)BWD_WAVES * 16 * TILE_LD) * sizeof(T);
 }
 
+// Tile (ti, tj) of X^T Y (real) for columns of length len: A(i, k) =
+// X(k, ti*16 + i), B(k, j) = Y(k, tj*16 + j), X and Y column-major.
 template <typename T>
+__device__ __forceinline__ typename Mfma<T>::V4 tile_xty(const T *X, const T *Y, const int len, const int n,
+                                                         const int ti, const int tj, const int li, const int lk)
+{
+  using V4 = typename Mfma<T>::V4;
+  const int ca = ti * 16 + li, cb = tj * 16 + li;
+  const T *const xa = X + (long long)ca * len, *const yb = Y + (long long)cb * len;
+  V4 acc = {0, 0, 0, 0};
+  // CH steps of 4 rows at a time: all their loads are issued before the
+  // chain of products that consumes them
+  for (int k0 = 0; k0 < len; k0 += 4 * CH) {
+    T a[CH], bv[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = k0 + 4 * c + lk;
+      a[c] = (ca < n && k < len) ? xa[k] : (T)0;
+      bv[c] = (cb < n && k < len) ? yb[k] : (T)0;
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      acc = Mfma<T>::op(a[c], bv[c], acc);
+  }
+  return acc;
+}
+
+// The same tile of X^H Y for complex X = Xr + i Xi, Y = Yr + i Yi:
+// re = Xr^T Yr + Xi^T Yi, im = Xr^T Yi - Xi^T Yr.
+template <typename T>
+__device__ __forceinline__ void tile_xhy(const T *Xr, const T *Xi, const T *Yr, const T *Yi, const int len, const int n,
+                                         const int ti, const int tj, const int li, const int lk,
+                                         typename Mfma<T>::V4 &re, typename Mfma<T>::V4 &im)
+{
+  const int ca = ti * 16 + li, cb = tj * 16 + li;
+  const long long oa = (long long)ca * len, ob = (long long)cb * len;
+  re = {0, 0, 0, 0};
+  im = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < len; k0 += 4 * (CH / 2)) {
+    T ar[CH / 2], ai[CH / 2], br[CH / 2], bi[CH / 2];
+#pragma unroll
+    for (int c = 0; c < CH / 2; ++c) {
+      const int k = k0 + 4 * c + lk;
+      const bool ia = (ca < n && k < len), ib = (cb < n && k < len);
+      ar[c] = ia ? Xr[oa + k] : (T)0;
+      ai[c] = ia ? Xi[oa + k] : (T)0;
+      br[c] = ib ? Yr[ob + k] : (T)0;
+      bi[c] = ib ? Yi[ob + k] : (T)0;
+    }
+#pragma unroll
+    for (int c = 0; c < CH / 2; ++c) {
+      re = Mfma<T>::op(ar[c], br[c], re);
+      re = Mfma<T>::op(ai[c], bi[c], re);
+      im = Mfma<T>::op(ar[c], bi[c], im);
+      im = Mfma<T>::op(-ai[c], br[c], im);
+    }
+  }
+}
+
+// Stores a 16 x 16 result tile (rows r0.., column c0..) of an m x n
+// column-major output through the wave's LDS tile, so that each column's 16
+// rows go out as one contiguous run.
+template <typename T>
+__device__ __forceinline__ void store_tile(T *out, const int m, const int n, const int r0, const int c0,
+                                           const typename Mfma<T>::V4 &acc, T *sT, const int lane)
+{
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    sT[li * TILE_LD + Mfma<T>::row(lk, r)] = acc[r];
+  wave_sync();
+  const int cc = lane >> 2, rr = 4 * (lane & 3);
+  const int col = c0 + cc;
+  if (col < n) {
+    T *const o = out + (long long)col * m + r0 + rr;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (r0 + rr + q < m)
+        o[q] = sT[cc * TILE_LD + rr + q];
+  }
+  wave_sync();
+}
+
+template <typename T, bool GW>
 __global__ __launch_bounds__(64 * BWD_WAVES) void k_svd_bwd(const int nb, const int m, const int n, const int n16,
                                                              const T *__restrict__ U, const T *__restrict__ s,
                                                              const T *__restrict__ V, const T *__restrict__ dU,
                                                              const T *__restrict__ ds, const T *__restrict__ dV,
-                                                             T *__restrict__ dA)
+                                                             T *__restrict__ dA, T *__restrict__ ws)
 {
   // per matrix b: U, dU, dA are m x n column-major (ld m) at b*m*n; V, dV are
   // n x n (ld n) at b*n*n; s, ds at b*n.
   using V4 = typename Mfma<T>::V4;
   extern __shared__ double smem_[];
   const int ld = n16 + 1;
-  T *const sJ = (T *)smem_;          // J, then N
-  T *const sK = sJ + n16 * ld;       // K, then P
-  T *const sR = sK + n16 * ld;       // 1/s_k (0 past n)
+  T *const sJ = GW ? ws + (long long)blockIdx.x * 2 * n16 * ld : (T *)smem_; // J, then N
+  T *const sK = sJ + n16 * ld;                                               // K, then P
+  T *const sR = GW ? (T *)smem_ : sK + n16 * ld;                            // 1/s_k (0 past n)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   T *const sT = sR + n16 + w * 16 * TILE_LD;
   const int li = lane & 15, lk = lane >> 4;
@@ -96,34 +181,14 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void k_svd_bwd(const int nb, const 
     const T *const sb = s + (long long)b * n, *const dsb = ds + (long long)b * n;
     for (int t = threadIdx.x; t < n16; t += 64 * BWD_WAVES)
       sR[t] = (t < n) ? (T)1 / sb[t] : (T)0;
-    // J = U^T dU (inner m) and K = V^T dV (inner n): tile (ti, tj) of either
-    // is A(i, k) = X(k, ti*16 + i), B(k, j) = Y(k, tj*16 + j) with X, Y
-    // column-major of column length len
+    // J = U^T dU (inner m) and K = V^T dV (inner n)
     for (int task = w; task < 2 * nt * nt; task += BWD_WAVES) {
       const bool isK = task >= nt * nt;
       const int tt = isK ? task - nt * nt : task;
       const int ti = tt % nt, tj = tt / nt;
-      const T *const X = isK ? Vb : Ub;
-      const T *const Y = isK ? dVb : dUb;
-      const int len = isK ? n : m;
-      const int ca = ti * 16 + li, cb = tj * 16 + li;
-      const T *const xa = X + (long long)ca * len, *const yb = Y + (long long)cb * len;
-      V4 acc = {0, 0, 0, 0};
-      // CH steps of 4 rows at a time: all their loads are issued before the
-      // chain of products that consumes them
-      for (int k0 = 0; k0 < len; k0 += 4 * CH) {
-        T a[CH], bv[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          const int k = k0 + 4 * c + lk;
-          a[c] = (ca < n && k < len) ? xa[k] : (T)0;
-          bv[c] = (cb < n && k < len) ? yb[k] : (T)0;
-        }
-#pragma unroll
-        for (int c = 0; c < CH; ++c)
-          acc = Mfma<T>::op(a[c], bv[c], acc);
-      }
+      const V4 acc = isK ? tile_xty<T>(Vb, dVb, n, n, ti, tj, li, lk) : tile_xty<T>(Ub, dUb, m, n, ti, tj, li, lk);
       T *const dst = isK ? sK : sJ;
+      const int cb = tj * 16 + li;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         dst[cb * ld + ti * 16 + Mfma<T>::row(lk, r)] = acc[r];
@@ -154,7 +219,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void k_svd_bwd(const int nb, const 
       sJ[i * ld + j] = F * (aJ * si + sj * aK) - Jji * sR[i];
     }
     __syncthreads();
-    // P = N V^T in place of K: A(i, k) = N(ti*16 + i, k) from LDS, B(k, c) =
+    // P = N V^T in place of K: A(i, k) = N(ti*16 + i, k), B(k, c) =
     // V(tj*16 + c, k) = V[k n + c], the 16 entries of a row contiguous
     for (int tt = w; tt < nt * nt; tt += BWD_WAVES) {
       const int ti = tt % nt, tj = tt / nt;
@@ -201,25 +266,185 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void k_svd_bwd(const int nb, const 
           acc = Mfma<T>::op(a2[c] * sR[k0 + 4 * c + lk], b2[c], acc);
         }
       }
-      // through the wave's tile: lane l then stores rows 4(l%4)..+3 of
-      // column l/4, so each column's 16 rows go out as one contiguous run
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        sT[li * TILE_LD + Mfma<T>::row(lk, r)] = acc[r];
-      wave_sync();
-      const int cc = lane >> 2, rr = 4 * (lane & 3);
-      const int col = cj * 16 + cc;
-      if (col < n) {
-        T *const out = dA + (long long)b * m * n + (long long)col * m + ri * 16 + rr;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (ri * 16 + rr + q < m)
-            out[q] = sT[cc * TILE_LD + rr + q];
-      }
-      wave_sync();
+      store_tile<T>(dA + (long long)b * m * n, m, n, ri * 16, cj * 16, acc, sT, lane);
     }
     __syncthreads();
   }
+}
+
+// Complex adjoint (split real and imaginary planes), for a loss that does
+// not depend on the phase of the singular vector pairs (u_k, v_k) ->
+// (u_k e^{i phi}, v_k e^{i phi}).  With J = U^H dU, K = V^H dV:
+//   N_ij = F_ij ((J - J^H)_ij s_j + s_i (K - K^H)_ij) - J_ij / s_j  (i != j),
+//   N_ii = ds_i - J_ii / s_i + i (Im J_ii - Im K_ii) / (2 s_i),
+//   P = N V^H,  dA = U P + (dU S^-1) V^H,
+// each complex product as four real ones on the matrix cores (a difference
+// by a negated operand, which is exact).  The diagonal phase term is the one
+// the gauge leaves: only the difference of the phase changes of u_k and v_k
+// is determined by dA, and for a phase-independent loss Im K_ii = -Im J_ii.
+template <typename T, bool GW>
+__global__ __launch_bounds__(64 * BWD_WAVES) void k_csvd_bwd(const int nb, const int m, const int n, const int n16,
+                                                              const T *__restrict__ Ur, const T *__restrict__ Ui,
+                                                              const T *__restrict__ s, const T *__restrict__ Vr,
+                                                              const T *__restrict__ Vi, const T *__restrict__ dUr,
+                                                              const T *__restrict__ dUi, const T *__restrict__ ds,
+                                                              const T *__restrict__ dVr, const T *__restrict__ dVi,
+                                                              T *__restrict__ dAr, T *__restrict__ dAi,
+                                                              T *__restrict__ ws)
+{
+  using V4 = typename Mfma<T>::V4;
+  extern __shared__ double smem_[];
+  const int ld = n16 + 1, sz = n16 * ld;
+  T *const sJr = GW ? ws + (long long)blockIdx.x * 4 * sz : (T *)smem_; // J, then N
+  T *const sJi = sJr + sz;
+  T *const sKr = sJi + sz;                                              // K, then P
+  T *const sKi = sKr + sz;
+  T *const sR = GW ? (T *)smem_ : sKi + sz;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T *const sT = sR + n16 + w * 16 * TILE_LD;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nt = n16 >> 4, mt = (m + 15) >> 4;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const long long oU = (long long)b * m * n, oV = (long long)b * n * n;
+    const T *const sb = s + (long long)b * n, *const dsb = ds + (long long)b * n;
+    for (int t = threadIdx.x; t < n16; t += 64 * BWD_WAVES)
+      sR[t] = (t < n) ? (T)1 / sb[t] : (T)0;
+    for (int task = w; task < 2 * nt * nt; task += BWD_WAVES) {
+      const bool isK = task >= nt * nt;
+      const int tt = isK ? task - nt * nt : task;
+      const int ti = tt % nt, tj = tt / nt;
+      V4 re, im;
+      if (isK)
+        tile_xhy<T>(Vr + oV, Vi + oV, dVr + oV, dVi + oV, n, n, ti, tj, li, lk, re, im);
+      else
+        tile_xhy<T>(Ur + oU, Ui + oU, dUr + oU, dUi + oU, m, n, ti, tj, li, lk, re, im);
+      T *const dr = isK ? sKr : sJr, *const di = isK ? sKi : sJi;
+      const int cb = tj * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        dr[cb * ld + ti * 16 + Mfma<T>::row(lk, r)] = re[r];
+        di[cb * ld + ti * 16 + Mfma<T>::row(lk, r)] = im[r];
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n16 * n16; t += 64 * BWD_WAVES) {
+      const int i = t % n16, j = t / n16;
+      if (i > j)
+        continue;
+      const int ij = j * ld + i, ji = i * ld + j;
+      if (j >= n) {
+        sJr[ij] = sJi[ij] = sJr[ji] = sJi[ji] = (T)0;
+        continue;
+      }
+      if (i == j) {
+        const T r = sR[i];
+        const T jr = sJr[ij], ji_ = sJi[ij];
+        sJr[ij] = dsb[i] - jr * r;
+        sJi[ij] = (T)0.5 * (ji_ - sKi[ij]) * r - ji_ * r;
+        continue;
+      }
+      const T si = sb[i], sj = sb[j];
+      const T F = (T)1 / ((sj - si) * (sj + si));
+      const T Jr_ij = sJr[ij], Ji_ij = sJi[ij], Jr_ji = sJr[ji], Ji_ji = sJi[ji];
+      // (J - J^H)_ij = (Jr_ij - Jr_ji) + i (Ji_ij + Ji_ji); (J - J^H)_ji is its
+      // negated conjugate and F_ji = -F_ij, so N_ji takes F (ajr s_i + s_j akr)
+      // as real part and -F (aji s_i + s_j aki) as imaginary part
+      const T ajr = Jr_ij - Jr_ji, aji = Ji_ij + Ji_ji;
+      const T akr = sKr[ij] - sKr[ji], aki = sKi[ij] + sKi[ji];
+      sJr[ij] = F * (ajr * sj + si * akr) - Jr_ij * sR[j];
+      sJi[ij] = F * (aji * sj + si * aki) - Ji_ij * sR[j];
+      sJr[ji] = F * (ajr * si + sj * akr) - Jr_ji * sR[i];
+      sJi[ji] = -F * (aji * si + sj * aki) - Ji_ji * sR[i];
+    }
+    __syncthreads();
+    // P = N V^H in place of K: B(k, c) = conj(V(tj*16 + c, k)), so
+    // Pr = Nr Vr^T + Ni Vi^T and Pi = Ni Vr^T - Nr Vi^T
+    for (int tt = w; tt < nt * nt; tt += BWD_WAVES) {
+      const int ti = tt % nt, tj = tt / nt;
+      const int ia = ti * 16 + li, cb = tj * 16 + li;
+      V4 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
+      for (int k0 = 0; k0 < n16; k0 += 16) {
+        T ar[4], ai[4], br[4], bi[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int k = k0 + 4 * c + lk;
+          const bool in = (k < n && cb < n);
+          ar[c] = sJr[k * ld + ia];
+          ai[c] = sJi[k * ld + ia];
+          br[c] = in ? Vr[oV + (long long)k * n + cb] : (T)0;
+          bi[c] = in ? Vi[oV + (long long)k * n + cb] : (T)0;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          pr = Mfma<T>::op(ar[c], br[c], pr);
+          pr = Mfma<T>::op(ai[c], bi[c], pr);
+          pi = Mfma<T>::op(ai[c], br[c], pi);
+          pi = Mfma<T>::op(-ar[c], bi[c], pi);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        sKr[cb * ld + ti * 16 + Mfma<T>::row(lk, r)] = pr[r];
+        sKi[cb * ld + ti * 16 + Mfma<T>::row(lk, r)] = pi[r];
+      }
+    }
+    __syncthreads();
+    // dA = U P + (dU S^-1) V^H:
+    //   re = Ur Pr - Ui Pi + (dUr S^-1) Vr^T + (dUi S^-1) Vi^T
+    //   im = Ur Pi + Ui Pr + (dUi S^-1) Vr^T - (dUr S^-1) Vi^T
+    for (int tt = w; tt < mt * nt; tt += BWD_WAVES) {
+      const int ri = tt % mt, cj = tt / mt;
+      const int ra = ri * 16 + li, cb = cj * 16 + li;
+      const bool rin = ra < m;
+      V4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+      for (int k0 = 0; k0 < n16; k0 += 16) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int k = k0 + 4 * c + lk;
+          const bool kin = k < n;
+          const long long ou = oU + (long long)k * m + ra, ov = oV + (long long)k * n + cb;
+          const T ur = (rin && kin) ? Ur[ou] : (T)0, ui = (rin && kin) ? Ui[ou] : (T)0;
+          const T rs = sR[k0 + 4 * c + lk];
+          const T gr = (rin && kin) ? dUr[ou] * rs : (T)0, gi = (rin && kin) ? dUi[ou] * rs : (T)0;
+          const bool vin = (kin && cb < n);
+          const T vr = vin ? Vr[ov] : (T)0, vi = vin ? Vi[ov] : (T)0;
+          const T pr = sKr[cb * ld + k], pi = sKi[cb * ld + k];
+          re = Mfma<T>::op(ur, pr, re);
+          re = Mfma<T>::op(-ui, pi, re);
+          re = Mfma<T>::op(gr, vr, re);
+          re = Mfma<T>::op(gi, vi, re);
+          im = Mfma<T>::op(ur, pi, im);
+          im = Mfma<T>::op(ui, pr, im);
+          im = Mfma<T>::op(gi, vr, im);
+          im = Mfma<T>::op(-gr, vi, im);
+        }
+      }
+      store_tile<T>(dAr + oU, m, n, ri * 16, cj * 16, re, sT, lane);
+      store_tile<T>(dAi + oU, m, n, ri * 16, cj * 16, im, sT, lane);
+    }
+    __syncthreads();
+  }
+}
+
+// LDS-resident intermediates when they fit a CU's 160 KiB, else a global
+// workspace of one J/K block per workgroup (stream-ordered allocation,
+// at most two workgroups per CU).
+struct BwdLaunch {
+  int grid;
+  size_t lds;
+  bool gw;
+};
+
+template <typename T> static int bwd_shape(int nb, int n16, int pl, BwdLaunch &L)
+{
+  int dev = 0, cus = 0;
+  VJ_HIP(hipGetDevice(&dev));
+  VJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  L.gw = bwd_lds_bytes<T>(n16, pl, false) > 160 * 1024;
+  L.lds = bwd_lds_bytes<T>(n16, pl, L.gw);
+  const int per_cu = L.gw ? 2 : (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / L.lds));
+  L.grid = nb < per_cu * cus ? nb : per_cu * cus;
+  return 0;
 }
 
 template <typename T>
@@ -231,17 +456,50 @@ int svd_bwd(int nb, int m, int n, const T *U, const T *s, const T *V, const T *d
   if (n < 1 || m < n)
     return set_error(-2, "svd backward: need 1 <= n <= m");
   const int n16 = (n + 15) & ~15;
-  const size_t lds = bwd_lds_bytes<T>(n16);
-  if (lds > 160 * 1024)
-    return set_error(-2, "svd backward: n = %d needs %zu bytes of LDS (at most 160 KiB; n <= 96 for FP64)", n, lds);
-  int dev = 0, cus = 0;
-  VJ_HIP(hipGetDevice(&dev));
-  VJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
-  const int grid = nb < per_cu * cus ? nb : per_cu * cus;
-  if (lds > 64 * 1024)
-    VJ_HIP(hipFuncSetAttribute((const void *)k_svd_bwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_svd_bwd<T>, dim3(grid), dim3(64 * BWD_WAVES), lds, st, nb, m, n, n16, U, s, V, dU, ds, dV, dA);
+  BwdLaunch L{};
+  if (int rc = bwd_shape<T>(nb, n16, 1, L))
+    return rc;
+  T *ws = nullptr;
+  if (L.gw) {
+    VJ_HIP(hipMallocAsync((void **)&ws, (size_t)L.grid * bwd_jk_elems<T>(n16, 1) * sizeof(T), st));
+    hipLaunchKernelGGL((k_svd_bwd<T, true>), dim3(L.grid), dim3(64 * BWD_WAVES), L.lds, st, nb, m, n, n16, U, s, V,
+                       dU, ds, dV, dA, ws);
+    const int rc = check_launch();
+    VJ_HIP(hipFreeAsync(ws, st));
+    return rc;
+  }
+  if (L.lds > 64 * 1024)
+    VJ_HIP(hipFuncSetAttribute((const void *)k_svd_bwd<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds));
+  hipLaunchKernelGGL((k_svd_bwd<T, false>), dim3(L.grid), dim3(64 * BWD_WAVES), L.lds, st, nb, m, n, n16, U, s, V, dU,
+                     ds, dV, dA, ws);
+  return check_launch();
+}
+
+template <typename T>
+int csvd_bwd(int nb, int m, int n, const T *Ur, const T *Ui, const T *s, const T *Vr, const T *Vi, const T *dUr,
+             const T *dUi, const T *ds, const T *dVr, const T *dVi, T *dAr, T *dAi, hipStream_t st)
+{
+  if (nb <= 0)
+    return 0;
+  if (n < 1 || m < n)
+    return set_error(-2, "svd backward: need 1 <= n <= m");
+  const int n16 = (n + 15) & ~15;
+  BwdLaunch L{};
+  if (int rc = bwd_shape<T>(nb, n16, 2, L))
+    return rc;
+  T *ws = nullptr;
+  if (L.gw) {
+    VJ_HIP(hipMallocAsync((void **)&ws, (size_t)L.grid * bwd_jk_elems<T>(n16, 2) * sizeof(T), st));
+    hipLaunchKernelGGL((k_csvd_bwd<T, true>), dim3(L.grid), dim3(64 * BWD_WAVES), L.lds, st, nb, m, n, n16, Ur, Ui, s,
+                       Vr, Vi, dUr, dUi, ds, dVr, dVi, dAr, dAi, ws);
+    const int rc = check_launch();
+    VJ_HIP(hipFreeAsync(ws, st));
+    return rc;
+  }
+  if (L.lds > 64 * 1024)
+    VJ_HIP(hipFuncSetAttribute((const void *)k_csvd_bwd<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds));
+  hipLaunchKernelGGL((k_csvd_bwd<T, false>), dim3(L.grid), dim3(64 * BWD_WAVES), L.lds, st, nb, m, n, n16, Ur, Ui, s,
+                     Vr, Vi, dUr, dUi, ds, dVr, dVi, dAr, dAi, ws);
   return check_launch();
 }
 
@@ -264,6 +522,20 @@ vjh_int vjh_ssvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const float *U, const flo
                      const float *dU, const float *ds, const float *dV, float *dA, void *stream)
 {
   return vj::svd_bwd<float>((int)nb, (int)m, (int)n, U, s, V, dU, ds, dV, dA, (hipStream_t)stream);
+}
+vjh_int vjh_zsvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const double *Ur, const double *Ui, const double *s,
+                     const double *Vr, const double *Vi, const double *dUr, const double *dUi, const double *ds,
+                     const double *dVr, const double *dVi, double *dAr, double *dAi, void *stream)
+{
+  return vj::csvd_bwd<double>((int)nb, (int)m, (int)n, Ur, Ui, s, Vr, Vi, dUr, dUi, ds, dVr, dVi, dAr, dAi,
+                              (hipStream_t)stream);
+}
+vjh_int vjh_csvd_bwd(vjh_int nb, vjh_int m, vjh_int n, const float *Ur, const float *Ui, const float *s,
+                     const float *Vr, const float *Vi, const float *dUr, const float *dUi, const float *ds,
+                     const float *dVr, const float *dVi, float *dAr, float *dAi, void *stream)
+{
+  return vj::csvd_bwd<float>((int)nb, (int)m, (int)n, Ur, Ui, s, Vr, Vi, dUr, dUi, ds, dVr, dVi, dAr, dAi,
+                             (hipStream_t)stream);
 }
 
 } // extern "C"
