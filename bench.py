@@ -250,6 +250,10 @@ def bench_bsvd(args, kind: str = "d"):
     # the set-up scan, V := I, the scaling and the epilogue; G in and U, V out
     # for a matrix held in LDS), counted by the kernel (vjh_bplan_traffic)
     alg = float(np.sum(r["xfer"].astype(np.float64))) * w
+    # the bytes the same pair evaluations would move in single steps (two
+    # columns loaded per evaluation): the block rounds' saving shows as the
+    # ratio of the two
+    step_bytes = _bsvd_alg_bytes(m, npad, r["evals"], tr, w)
     k_s = r["kernel_ms"] / 1e3
     achieved = alg / k_s / 1e9
     value = nb * args.steps / r["wall"]
@@ -268,7 +272,9 @@ def bench_bsvd(args, kind: str = "d"):
                    "pair_slots": int(np.sum((np.minimum(info, 998) + 1) * npad * (npad // 2)))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bsvd",
-                     "kernel_avg_ms": r["kernel_ms"], "alg_bytes_per_launch": alg},
+                     "kernel_avg_ms": r["kernel_ms"], "alg_bytes_per_launch": alg,
+                     "step_equivalent_bytes": step_bytes,
+                     "step_equivalent_frac": step_bytes / k_s / 1e9 / HBM_PEAK_GBS},
     }
     if nb == 16384 and args.waves == 0:
         t, src = _profiled_traffic(f"k_bsvd_{'fp64' if kind == 'd' else 'fp32'}_16384")

@@ -293,6 +293,12 @@ vjh_int vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream);
  * ordering of pjs(4, n).  A run equals the plain driver run with the same
  * table.  id 0 returns the current ordering; another id returns -2. */
 vjh_int vjh_bplan_order(void *plan, int id);
+/* Cost model of the plan's processing order (each launch takes its matrices
+ * from a queue sorted by decreasing estimated cost, so that the costliest
+ * start first): 0, m n^2 (default); 1, n^2 (8 + m/64); 2, n^2 (16 + m/64).
+ * model < 0 returns the current model; > 2 returns -2.  The results do not
+ * depend on it; the autotuner picks it (step 3). */
+vjh_int vjh_bplan_queue(void *plan, int model);
 /* Autotuner of a plan, on copies of the input G (Gi) made into the scratch
  * buffer Gw (Gwi), each launch timed on its own with HIP events on `stream`:
  * (1) for the split row count and half of it, each launch at 1, 2, 4 and 8
@@ -300,8 +306,9 @@ vjh_int vjh_bplan_order(void *plan, int id);
  * the lane-group launches also at 8 and 4 lanes per pair, keeping the split
  * and shapes of the fastest total; (2) for LDS budgets of 0, 24, 48, 96 and
  * 152 KiB, the two short launches at each shape, keeping the fastest
- * budget and shapes.  The choices stay in the plan
- * (vjh_bplan_config).  ms[4] (may be NULL) receives the run times of the
+ * budget and shapes; (3) for each queue cost model (vjh_bplan_queue), the
+ * whole plan twice, keeping the fastest model.  The choices stay in the
+ * plan (vjh_bplan_config, vjh_bplan_queue).  ms[4] (may be NULL) receives the run times of the
  * four counts at the default split, -1 for skipped counts.  V, eS, fS and
  * info are overwritten.  Returns the count of the first non-empty launch,
  * or a negative code.  Synchronises `stream`. */

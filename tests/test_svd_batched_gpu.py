@@ -26,12 +26,15 @@ def _batch(nb, seed, nmax=48, dtype=np.float64):
     return m, n, npad, offG, offV, offS, tot, G
 
 
-def _run(kind, m, npad, G, tot, waves=0, Gi=None, split=None, budget=None, wv=None, lanes=None, order=None):
+def _run(kind, m, npad, G, tot, waves=0, Gi=None, split=None, budget=None, wv=None, lanes=None, order=None,
+         queue=None):
     import torch
     dt = torch.float64 if kind in "dz" else torch.float32
     plan = BT.BatchedSVD(m, npad, kind=kind, waves=waves)
     if order is not None:
         assert plan.order(order) == order
+    if queue is not None:
+        assert plan.queue(queue) == queue
     if budget is not None:
         plan.set(split if split is not None else (64 if kind in "zc" else 128), budget, wv, lanes)
     Gd = torch.from_numpy(G.copy()).cuda()
@@ -374,3 +377,13 @@ def test_block_rounds_halve_column_traffic():
     ratio = per[6] / per[4]
     print("traffic per evaluation, ordering 6 / ordering 4:", np.round(ratio, 3))
     assert np.all(ratio < 0.75), ratio
+
+
+def test_queue_models_same_bits():
+    """The processing order's cost model (vjh_bplan_queue) changes only the
+    order in which workgroups take matrices: every output is the same."""
+    m, n, npad, offG, offV, offS, tot, G = _batch(40, 313, nmax=96)
+    runs = [_run("d", m, npad, G, tot, queue=q) for q in (0, 1, 2)]
+    for r in runs[1:]:
+        for f in ("G", "V", "eS", "fS", "info", "tr"):
+            assert np.array_equal(r[f], runs[0][f], equal_nan=True), f

@@ -38,6 +38,8 @@ def _declare(L):
     L.vjh_bplan_traffic.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.vjh_bplan_order.restype = C.c_int64
     L.vjh_bplan_order.argtypes = [C.c_void_p, C.c_int]
+    L.vjh_bplan_queue.restype = C.c_int64
+    L.vjh_bplan_queue.argtypes = [C.c_void_p, C.c_int]
     L.vjh_bplan_set.restype = C.c_int64
     L.vjh_bplan_set.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p]
     L.vjh_bplan_config.restype = C.c_int64
@@ -100,7 +102,8 @@ class BatchedSVD:
         names = ("long", "short_global", "short_lds")
         return {"matrices": dict(zip(names, out[0:3].tolist())), "waves": dict(zip(names, out[3:6].tolist())),
                 "lanes_per_pair": dict(zip(names, out[6:9].tolist())), "split_rows": int(out[9]),
-                "lds_budget": int(out[10]), "lds_bytes_per_workgroup": int(out[11])}
+                "lds_budget": int(out[10]), "lds_bytes_per_workgroup": int(out[11]), "queue_model": self.queue(),
+                "ordering": self.order()}
 
     def set(self, split_rows: int, lds_budget: int, waves=None, lanes=None) -> None:
         """Fix the split row count, the LDS budget (bytes) and optionally the
@@ -160,6 +163,14 @@ class BatchedSVD:
         rc = _lib.lib().vjh_bplan_order(self.plan, int(id_))
         if rc < 0:
             raise ValueError(f"vjh_bplan_order: {_lib.last_error()}")
+        return int(rc)
+
+    def queue(self, model: int = -1) -> int:
+        """Set the cost model of the processing order (0, 1 or 2) and
+        return it; -1 only returns it (vjh_bplan_queue)."""
+        rc = _lib.lib().vjh_bplan_queue(self.plan, int(model))
+        if rc < 0:
+            raise ValueError(f"vjh_bplan_queue: {_lib.last_error()}")
         return int(rc)
 
     def close(self):

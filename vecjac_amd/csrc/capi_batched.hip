@@ -41,6 +41,7 @@ struct BPlan {
   long long totG = 0;      // elements of the packed G (max over b of offG + m n)
   unsigned swp = 999u;
   int ord = 6;             // ordering of the pairs (pjs id)
+  int queue = 0;           // cost model of the processing order (queue_cost)
   int dev = 0;
   std::vector<long long> hm, hn; // the shapes, for re-partitioning
   long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each)
@@ -70,8 +71,22 @@ int class_of(const BPlan *P, long long m, long long n)
   return (lds_matrix_bytes(m, n, cplx, tsize(P->kind)) <= P->lds_budget) ? 2 : 1;
 }
 
-// Processing order (the classes in turn, each by decreasing m n^2, the
-// costliest first), class counts and row maxima; uploads the order.
+// Estimated cost of a matrix under the plan's queue model: 0, m n^2 (the
+// column traffic of a sweep); 1 and 2, n^2 (c + m / 64) with c = 8 or 16 (a
+// pair evaluation costs a fixed decision stream plus a share per 64-row
+// chunk, the fixed part measured at about 15 chunks' worth).
+double queue_cost(const BPlan *P, long long m, long long n)
+{
+  const double md = (double)m, nd = (double)n;
+  switch (P->queue) {
+  case 1: return nd * nd * (8.0 + md / 64.0);
+  case 2: return nd * nd * (16.0 + md / 64.0);
+  default: return md * nd * nd;
+  }
+}
+
+// Processing order (the classes in turn, each by decreasing estimated cost,
+// the costliest first), class counts and row maxima; uploads the order.
 int partition(BPlan *P)
 {
   const long long nb = P->nb;
@@ -82,7 +97,7 @@ int partition(BPlan *P)
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
     if (cls[(size_t)x] != cls[(size_t)y])
       return cls[(size_t)x] < cls[(size_t)y];
-    return P->hm[(size_t)x] * P->hn[(size_t)x] * P->hn[(size_t)x] > P->hm[(size_t)y] * P->hn[(size_t)y] * P->hn[(size_t)y];
+    return queue_cost(P, P->hm[(size_t)x], P->hn[(size_t)x]) > queue_cost(P, P->hm[(size_t)y], P->hn[(size_t)y]);
   });
   for (int c = 0; c < NCLASS; ++c)
     P->count[c] = P->max_m_c[c] = 0;
@@ -278,6 +293,34 @@ int tune_shapes(BPlan *P, const void *G, const void *Gi, void *Gw, void *Gwi, vo
   return rc;
 }
 
+// One timed run of the whole plan (every launch) on a scratch copy of G.
+int time_plan(BPlan *P, const void *G, const void *Gi, void *Gw, void *Gwi, void *V, void *Vi, void *eS, void *fS,
+              vjh_int *info, hipStream_t s, float *ms)
+{
+  const bool cplx = (P->kind & 1) != 0;
+  const size_t bytes = (size_t)P->totG * (size_t)tsize(P->kind);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = set_error(VJ_EHIP, "autotune: event creation failed");
+  if (!rc && bytes && hipMemcpyAsync(Gw, G, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    rc = set_error(VJ_EHIP, "autotune: copy failed");
+  if (!rc && bytes && cplx && hipMemcpyAsync(Gwi, Gi, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    rc = set_error(VJ_EHIP, "autotune: copy failed");
+  if (!rc && hipEventRecord(e0, s) != hipSuccess)
+    rc = set_error(VJ_EHIP, "autotune: event failed");
+  if (!rc)
+    rc = run_any(P, Gw, Gwi, V, Vi, eS, fS, info, nullptr, s, nullptr, 7u);
+  if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+    rc = set_error(VJ_EHIP, "autotune: event failed");
+  if (e0)
+    (void)hipEventDestroy(e0);
+  if (e1)
+    (void)hipEventDestroy(e1);
+  return rc;
+}
+
 } // namespace
 
 extern "C" {
@@ -440,6 +483,21 @@ vjh_int vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream)
   return 0;
 }
 
+vjh_int vjh_bplan_queue(void *plan, int model)
+{
+  BPlan *P = (BPlan *)plan;
+  if (!P)
+    return set_error(-1, "null plan");
+  if (model < 0)
+    return P->queue;
+  if (model > 2)
+    return set_error(-2, "batched SVD: queue cost model %d (0, 1 or 2)", model);
+  P->queue = model;
+  if (int rc = partition(P))
+    return rc;
+  return model;
+}
+
 vjh_int vjh_bplan_order(void *plan, int id)
 {
   BPlan *P = (BPlan *)plan;
@@ -525,6 +583,27 @@ vjh_int vjh_bplan_autotune(void *plan, const void *G, const void *Gi, void *Gw, 
   P->waves_c[2] = bw2;
   P->lg_c[1] = bl1;
   P->lg_c[2] = bl2;
+  // 3. the processing order: each queue cost model, the whole plan timed
+  //    twice at the chosen shapes, the fastest kept
+  int best_q = P->queue;
+  float best_q_ms = -1.0f;
+  for (int q = 0; q < 3; ++q) {
+    P->queue = q;
+    if (int rc = partition(P))
+      return rc;
+    for (int rep = 0; rep < 2; ++rep) {
+      float t = 0.0f;
+      if (int rc = time_plan(P, G, Gi, Gw, Gwi, V, Vi, eS, fS, info, s, &t))
+        return rc;
+      if (best_q_ms < 0.0f || t < best_q_ms) {
+        best_q_ms = t;
+        best_q = q;
+      }
+    }
+  }
+  P->queue = best_q;
+  if (int rc = partition(P))
+    return rc;
   return (vjh_int)vjh_bplan_waves(P);
 }
 
