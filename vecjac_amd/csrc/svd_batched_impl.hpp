@@ -536,6 +536,47 @@ __device__ __forceinline__ unsigned pair2(T (&x0)[E], T (&y0)[E], T (&x1)[E], T 
   return ch;
 }
 
+// Stores the changed columns of a unit after its round and records the step
+// of each G column's last change; ch0 and ch1 are the change bits (pair2's
+// form) of its two steps.  Returns the elements moved (loads included).
+template <typename T, int E>
+__device__ __forceinline__ unsigned long long unit_store(T *Gr, T *Vr, const int m, const int n, const int nE,
+                                                         const int lane, const Step<T> &S, int *mod, const int g0,
+                                                         const unsigned I, const unsigned J, const unsigned ch0,
+                                                         const unsigned ch1, T (&ga)[E], T (&gb)[E], T (&gc)[E],
+                                                         T (&gd)[E], T (&va)[2], T (&vb)[2], T (&vc)[2], T (&vd)[2])
+{
+  const bool diag = (I == J);
+  const int g1 = g0 + 1;
+  const unsigned cA = 2u * I, cB = 2u * I + 1u, cC = 2u * J, cD = 2u * J + 1u;
+  // changes per register column: a by pair 0 at both steps, b by pair 1
+  // at both, c by pair 0 then 1, d by pair 1 then 0 (for a block met by
+  // itself d holds column 2I+1, changed by pair 0 at both steps)
+  const unsigned s0a = ch0 & 5u, s1a = ch1 & 5u, s0b = (ch0 >> 1) & 5u, s1b = (ch1 >> 1) & 5u;
+  const unsigned chA[4] = {s0a | s1a, s0b | s1b, s0a | s1b, diag ? (s0a | s1a) : (s0b | s1a)};
+  const bool last1[4] = {(s1a & 1u) != 0u, (s1b & 1u) != 0u, (s1b & 1u) != 0u, (s1a & 1u) != 0u};
+  const unsigned col[4] = {cA, cB, cC, diag ? cB : cD};
+  unsigned long long xf = (unsigned long long)(diag ? 2 : 4) * (unsigned long long)(m + n);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (diag && (k == 1 || k == 2))
+      continue;
+    T (&g)[E] = (k == 0) ? ga : (k == 1) ? gb : (k == 2) ? gc : gd;
+    T (&v)[2] = (k == 0) ? va : (k == 1) ? vb : (k == 2) ? vc : vd;
+    if ((chA[k] & 1u) || S.sG) {
+      store_col<T, E>(Gr, m, nE, col[k], lane, g);
+      if (lane == 0)
+        mod[col[k]] = last1[k] ? g1 : g0;
+      xf += (unsigned long long)m;
+    }
+    if ((chA[k] & 4u) || S.sV) {
+      store_col<T, 2>(Vr, n, 2, col[k], lane, v);
+      xf += (unsigned long long)n;
+    }
+  }
+  return xf;
+}
+
 // Register footprint of the block form: the next unit is prefetched while
 // its four G columns take at most 8 registers each.
 template <typename T, int E> struct BlockShape {
@@ -592,7 +633,6 @@ __device__ __forceinline__ void block_round(T *Gr, T *Vr, T *eS, T *fS, const in
     unit_of(0, I, J);
     load_unit(I, J, ga, gb, gc, gd, va, vb, vc, vd);
   }
-  const int g1 = g0 + 1;
   for (int i = 0; i < cnt; ++i) {
     unsigned In = 0u, Jn = 0u;
     T na[E], nbb[E], nc[E], nd[E], nva[2], nvb[2], nvc[2], nvd[2];
@@ -638,32 +678,7 @@ __device__ __forceinline__ void block_round(T *Gr, T *Vr, T *eS, T *fS, const in
     if (werr)
       return;
     wev += diag ? 2ull : 4ull;
-    // changes per register column: a by pair 0 at both steps, b by pair 1
-    // at both, c by pair 0 then 1, d by pair 1 then 0 (for a block met by
-    // itself d holds column 2I+1, changed by pair 0 at both steps)
-    const unsigned s0a = ch0 & 5u, s1a = ch1 & 5u, s0b = (ch0 >> 1) & 5u, s1b = (ch1 >> 1) & 5u;
-    const unsigned chA[4] = {s0a | s1a, s0b | s1b, s0a | s1b, diag ? (s0a | s1a) : (s0b | s1a)};
-    const bool last1[4] = {(s1a & 1u) != 0u, (s1b & 1u) != 0u, (s1b & 1u) != 0u, (s1a & 1u) != 0u};
-    const unsigned col[4] = {cA, cB, cC, diag ? cB : cD};
-    unsigned long long xf = (unsigned long long)(diag ? 2 : 4) * (unsigned long long)(m + n);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (diag && (k == 1 || k == 2))
-        continue;
-      T (&g)[E] = (k == 0) ? ga : (k == 1) ? gb : (k == 2) ? gc : gd;
-      T (&v)[2] = (k == 0) ? va : (k == 1) ? vb : (k == 2) ? vc : vd;
-      if ((chA[k] & 1u) || S.sG) {
-        store_col<T, E>(Gr, m, nE, col[k], lane, g);
-        if (lane == 0)
-          mod[col[k]] = last1[k] ? g1 : g0;
-        xf += (unsigned long long)m;
-      }
-      if ((chA[k] & 4u) || S.sV) {
-        store_col<T, 2>(Vr, n, 2, col[k], lane, v);
-        xf += (unsigned long long)n;
-      }
-    }
-    wxf += xf;
+    wxf += unit_store<T, E>(Gr, Vr, m, n, nE, lane, S, mod, g0, I, J, ch0, ch1, ga, gb, gc, gd, va, vb, vc, vd);
     if constexpr (PF) {
       I = In;
       J = Jn;
@@ -673,6 +688,208 @@ __device__ __forceinline__ void block_round(T *Gr, T *Vr, T *eS, T *fS, const in
       }
       va[0] = nva[0]; va[1] = nva[1]; vb[0] = nvb[0]; vb[1] = nvb[1];
       vc[0] = nvc[0]; vc[1] = nvc[1]; vd[0] = nvd[0]; vd[1] = nvd[1];
+    }
+  }
+}
+
+// Four pairs of one step (two units of a wave), reduced and decided
+// together: lane r & 3 decides pair r, so that one decision stream serves
+// four pairs; pair r is applied when bit r of `valid` is set.  Returns bit r
+// when pair r changed its G columns and bit 4 + r when it changed its V
+// columns; werr as in pair2.
+template <typename T, int E>
+__device__ __forceinline__ unsigned pair4(T (&x0)[E], T (&y0)[E], T (&x1)[E], T (&y1)[E], T (&x2)[E], T (&y2)[E],
+                                          T (&x3)[E], T (&y3)[E], T (&v0)[2], T (&w0)[2], T (&v1)[2], T (&w1)[2],
+                                          T (&v2)[2], T (&w2)[2], T (&v3)[2], T (&w3)[2], const unsigned valid,
+                                          const unsigned (&pq)[8], const int nE, const int lane, const Step<T> &S,
+                                          T *eS, T *fS, T &wG, unsigned long long &wcnt, int &werr)
+{
+  typedef typename Acc<T>::A A;
+  constexpr int B = 64;
+  T zi[E], vzi[2]; // imaginary parts of real data
+#pragma unroll
+  for (int j = 0; j < E; ++j)
+    zi[j] = (T)0;
+  vzi[0] = vzi[1] = (T)0;
+  T mx0, my0, mx1, my1, mx2, my2, mx3, my3;
+  pair_maxima<T, false, E>(x0, y0, zi, zi, nE, mx0, my0);
+  pair_maxima<T, false, E>(x1, y1, zi, zi, nE, mx1, my1);
+  pair_maxima<T, false, E>(x2, y2, zi, zi, nE, mx2, my2);
+  pair_maxima<T, false, E>(x3, y3, zi, zi, nE, mx3, my3);
+  red_max4(mx0, my0, mx1, my1);
+  red_max4(mx2, my2, mx3, my3);
+  const int kx0 = max_exp(mx0), ky0 = max_exp(my0), kx1 = max_exp(mx1), ky1 = max_exp(my1);
+  const int kx2 = max_exp(mx2), ky2 = max_exp(my2), kx3 = max_exp(mx3), ky3 = max_exp(my3);
+  DD<A> sx0{0, 0}, sy0{0, 0}, dr0{0, 0}, sx1{0, 0}, sy1{0, 0}, dr1{0, 0};
+  DD<A> sx2{0, 0}, sy2{0, 0}, dr2{0, 0}, sx3{0, 0}, sy3{0, 0}, dr3{0, 0}, di{0, 0};
+  pair_sums<T, false, E>(x0, y0, zi, zi, nE, kx0, ky0, sx0, sy0, dr0, di);
+  pair_sums<T, false, E>(x1, y1, zi, zi, nE, kx1, ky1, sx1, sy1, dr1, di);
+  red_dd4<A, B>(sx0, sy0, dr0, sx1, nullptr);
+  pair_sums<T, false, E>(x2, y2, zi, zi, nE, kx2, ky2, sx2, sy2, dr2, di);
+  red_dd4<A, B>(sy1, dr1, sx2, sy2, nullptr);
+  pair_sums<T, false, E>(x3, y3, zi, zi, nE, kx3, ky3, sx3, sy3, dr3, di);
+  red_dd4<A, B>(dr2, sx3, sy3, dr3, nullptr);
+  const int rl = lane & 3;
+  const bool b0 = (rl & 1) != 0, b1 = (rl & 2) != 0;
+  auto sel = [&](auto a0, auto a1, auto a2, auto a3) { return b1 ? (b0 ? a3 : a2) : (b0 ? a1 : a0); };
+  auto seld = [&](DD<A> a0, DD<A> a1, DD<A> a2, DD<A> a3) {
+    return dd_sel(b1, dd_sel(b0, a3, a2), dd_sel(b0, a1, a0));
+  };
+  const DD<A> dz{0, 0};
+  const PairScal<T> Pl = pair_scal<T, false>(sel(mx0, mx1, mx2, mx3), sel(my0, my1, my2, my3),
+                                             sel(kx0, kx1, kx2, kx3), sel(ky0, ky1, ky2, ky3),
+                                             seld(sx0, sx1, sx2, sx3), seld(sy0, sy1, sy2, sy3),
+                                             seld(dr0, dr1, dr2, dr3), dz);
+  const PairDec<T> Dl = pair_decide<T, false>(Pl, S.tol, S.gst);
+  unsigned ch = 0u;
+  auto apply = [&](T (&x)[E], T (&y)[E], T (&v)[2], T (&w)[2], const int r) {
+    const PairDec<T> D = lane_dec(Dl, r);
+    const bool gsr = (D.code == 3) || (D.code == -3);
+    const PairScal<T> P = gsr ? lane_scal(Pl, r) : Pl;
+    T eSp, fSp, eSq, fSq;
+    const PairOut<T, false> o = pair_apply<T, false, B, E, false>(0, x, y, zi, zi, nullptr, nullptr, nullptr, nullptr,
+                                                                  P, D, 0, eSp, fSp, eSq, fSq, nullptr, lane, nE);
+    int err = D.err;
+    if (!err && (__ballot(!isfinite_(o.tG)) != 0ull))
+      err = -580;
+    if (err) {
+      werr = err;
+      return;
+    }
+    if (D.code != 1)
+      ch |= 1u << r;
+    if ((D.code != 1) && (D.code != 3)) {
+      v_update<T, false>(D, v, w, vzi, vzi);
+      ch |= 16u << r;
+    }
+    if (lane == r) {
+      eS[pq[2 * r]] = eSp; fS[pq[2 * r]] = fSp;
+      eS[pq[2 * r + 1]] = eSq; fS[pq[2 * r + 1]] = fSq;
+    }
+    wG = xmax(wG, o.tG);
+    wcnt += (unsigned long long)o.trans;
+  };
+  if (valid & 1u)
+    apply(x0, y0, v0, w0, 0);
+  if ((valid & 2u) && !werr)
+    apply(x1, y1, v1, w1, 1);
+  if ((valid & 4u) && !werr)
+    apply(x2, y2, v2, w2, 2);
+  if ((valid & 8u) && !werr)
+    apply(x3, y3, v3, w3, 3);
+  return ch;
+}
+
+// Block round with two units at a time per wave (short columns, up to 4
+// row chunks): the four pairs of a step share one decision stream (pair4),
+// so that a pair's decision costs a quarter of a pass instead of a half.
+// No prefetch (two units' columns already take the registers of one
+// prefetched unit); an odd last unit runs with an unapplied copy of itself.
+template <typename T, int E, int W>
+__device__ __forceinline__ void block_round2(T *Gr, T *Vr, T *eS, T *fS, const int m, const int n, const int nE,
+                                             const unsigned s, const int wave, const int lane, const Step<T> &S,
+                                             T &wG, unsigned long long &wcnt, int &werr, int *mod, const int g0,
+                                             unsigned long long &wev, unsigned long long &wxf)
+{
+  const unsigned nb = (unsigned)n >> 1u;
+  const int nu = (int)br_units(nb, s);
+  if (wave >= nu)
+    return;
+  const int cnt0 = (nu - wave + W - 1) / W;
+  int cnt = cnt0, slot = lane;
+  if ((S.sG == 0) && (S.sV == 0) && (g0 > n)) {
+    bool need = false;
+    if (lane < cnt0) {
+      unsigned I, J;
+      br_unit(nb, s, (unsigned)(wave + W * lane), I, J);
+      const int lim = g0 - n;
+      need = (mod[2u * I] >= lim) || (mod[2u * I + 1u] >= lim) || (mod[2u * J] >= lim) || (mod[2u * J + 1u] >= lim);
+    }
+    const unsigned long long mk = __ballot(need);
+    cnt = __popcll(mk);
+    if (cnt == 0)
+      return;
+    const int below = __popcll(mk & ((1ull << lane) - 1ull));
+    const int pos = need ? below : (cnt + lane - below);
+    slot = __builtin_amdgcn_ds_permute(pos << 2, lane);
+  }
+  auto unit_of = [&](const int i, unsigned &I, unsigned &J) {
+    const int k = __builtin_amdgcn_readlane(slot, (i < cnt) ? i : (cnt - 1));
+    br_unit(nb, s, (unsigned)(wave + W * k), I, J);
+  };
+  T zi[E], vzi[2];
+  auto load_unit = [&](const unsigned I, const unsigned J, T (&a)[E], T (&b)[E], T (&c)[E], T (&d)[E], T (&va)[2],
+                       T (&vb)[2], T (&vc)[2], T (&vd)[2]) {
+    load_pair<T, false, E>(Gr, nullptr, m, nE, (int)(2u * I), (int)(2u * I + 1u), lane, a, b, zi, zi);
+    load_pair<T, false, E>(Gr, nullptr, m, nE, (int)(2u * J), (int)(2u * J + 1u), lane, c, d, zi, zi);
+    load_vpair<T, false>(Vr, nullptr, n, (int)(2u * I), (int)(2u * I + 1u), lane, va, vb, vzi, vzi);
+    load_vpair<T, false>(Vr, nullptr, n, (int)(2u * J), (int)(2u * J + 1u), lane, vc, vd, vzi, vzi);
+    ready_pair<T, false, E>(m, nE, S.sG, lane, a, b, zi, zi);
+    ready_pair<T, false, E>(m, nE, S.sG, lane, c, d, zi, zi);
+    if (S.sV) {
+      scale_vpair<T>(S.sV, va, vb, vzi, vzi);
+      scale_vpair<T>(S.sV, vc, vd, vzi, vzi);
+    }
+  };
+  for (int i = 0; i < cnt; i += 2) {
+    const bool hasB = (i + 1 < cnt);
+    unsigned Ia, Ja, Ib, Jb;
+    unit_of(i, Ia, Ja);
+    unit_of(i + 1, Ib, Jb); // the last unit again when there is no second one
+    T ga[E], gb[E], gc[E], gd[E], va[2], vb[2], vc[2], vd[2];
+    T ha[E], hb[E], hc[E], hd[E], ua[2], ub[2], uc[2], ud[2];
+    load_unit(Ia, Ja, ga, gb, gc, gd, va, vb, vc, vd);
+    load_unit(Ib, Jb, ha, hb, hc, hd, ua, ub, uc, ud);
+    const bool dA = (Ia == Ja), dB = (Ib == Jb);
+    if (dA) {
+#pragma unroll
+      for (int j = 0; j < E; ++j)
+        gc[j] = gb[j];
+      vc[0] = vb[0];
+      vc[1] = vb[1];
+    }
+    if (dB) {
+#pragma unroll
+      for (int j = 0; j < E; ++j)
+        hc[j] = hb[j];
+      uc[0] = ub[0];
+      uc[1] = ub[1];
+    }
+    const unsigned valid = 1u | (dA ? 0u : 2u) | (hasB ? (4u | (dB ? 0u : 8u)) : 0u);
+    const unsigned aA = 2u * Ia, bA = aA + 1u, cA = 2u * Ja, dA_ = cA + 1u;
+    const unsigned aB = 2u * Ib, bB = aB + 1u, cB = 2u * Jb, dB_ = cB + 1u;
+    const unsigned pq0[8] = {aA, dA ? bA : cA, bA, dA_, aB, dB ? bB : cB, bB, dB_};
+    const unsigned c0 = pair4<T, E>(ga, gc, gb, gd, ha, hc, hb, hd, va, vc, vb, vd, ua, uc, ub, ud, valid, pq0, nE, lane,
+                                    S, eS, fS, wG, wcnt, werr);
+    if (werr)
+      return;
+    if (dA) {
+#pragma unroll
+      for (int j = 0; j < E; ++j)
+        gd[j] = gc[j];
+      vd[0] = vc[0];
+      vd[1] = vc[1];
+    }
+    if (dB) {
+#pragma unroll
+      for (int j = 0; j < E; ++j)
+        hd[j] = hc[j];
+      ud[0] = uc[0];
+      ud[1] = uc[1];
+    }
+    const unsigned pq1[8] = {aA, dA ? bA : dA_, bA, cA, aB, dB ? bB : dB_, bB, cB};
+    const unsigned c1 = pair4<T, E>(ga, gd, gb, gc, ha, hd, hb, hc, va, vd, vb, vc, ua, ud, ub, uc, valid, pq1, nE, lane,
+                                    S, eS, fS, wG, wcnt, werr);
+    if (werr)
+      return;
+    // the pair2 form of the change bits of each unit: G bits r, V bits 2 + r
+    const unsigned a0 = (c0 & 3u) | (((c0 >> 4) & 3u) << 2), a1 = (c1 & 3u) | (((c1 >> 4) & 3u) << 2);
+    const unsigned b0 = ((c0 >> 2) & 3u) | (((c0 >> 6) & 3u) << 2), b1 = ((c1 >> 2) & 3u) | (((c1 >> 6) & 3u) << 2);
+    wev += dA ? 2ull : 4ull;
+    wxf += unit_store<T, E>(Gr, Vr, m, n, nE, lane, S, mod, g0, Ia, Ja, a0, a1, ga, gb, gc, gd, va, vb, vc, vd);
+    if (hasB) {
+      wev += dB ? 2ull : 4ull;
+      wxf += unit_store<T, E>(Gr, Vr, m, n, nE, lane, S, mod, g0, Ib, Jb, b0, b1, ha, hb, hc, hd, ua, ub, uc, ud);
     }
   }
 }
@@ -1145,7 +1362,8 @@ __device__ __forceinline__ void wave_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS,
 }
 
 // Block rounds of the wave form (real data, pjs id 6), with the register
-// footprint of this matrix's row count.
+// footprint of this matrix's row count: two units at a time up to 4 row
+// chunks (block_round2), one prefetched or single unit beyond.
 template <typename T, int E, int W>
 __device__ __forceinline__ void block_dispatch(T *Gr, T *Vr, T *eS, T *fS, const int m, const int n, const unsigned s,
                                                const int wave, const int lane, const Step<T> &S, T &wG,
@@ -1156,14 +1374,16 @@ __device__ __forceinline__ void block_dispatch(T *Gr, T *Vr, T *eS, T *fS, const
 #define VJ_BR(EE)                                                                                                     \
   block_round<T, EE, W, BlockShape<T, EE>::PF>(Gr, Vr, eS, fS, m, n, nE, s, wave, lane, S, wG, wcnt, werr, mod, g0,  \
                                                wev, wxf)
+#define VJ_BR2(EE)                                                                                                    \
+  block_round2<T, EE, W>(Gr, Vr, eS, fS, m, n, nE, s, wave, lane, S, wG, wcnt, werr, mod, g0, wev, wxf)
   if ((E < 4) || (nE <= 2)) {
     constexpr int E2 = (E >= 2) ? 2 : E;
-    VJ_BR(E2);
+    VJ_BR2(E2);
     return;
   }
   if constexpr (E >= 4) {
     if ((E < 8) || (nE <= 4)) {
-      VJ_BR(4);
+      VJ_BR2(4);
       return;
     }
   }
@@ -1176,6 +1396,7 @@ __device__ __forceinline__ void block_dispatch(T *Gr, T *Vr, T *eS, T *fS, const
   if constexpr (E >= 16)
     VJ_BR(E);
 #undef VJ_BR
+#undef VJ_BR2
 }
 
 // Minimum waves per SIMD the register allocation must allow: two, which keeps
@@ -1449,10 +1670,17 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
   }
 }
 
+// Dynamic LDS of a launch: the resident matrices of the LDS form.
+template <typename T, bool CPLX, int E, int W, bool GRP, bool LDS>
+static size_t launch_lds(const BatchArgs<T> &a)
+{
+  return LDS ? (size_t)a.lds_bytes : 0;
+}
+
 template <typename T, bool CPLX, int E, int W, bool GRP, bool LDS, int LG>
 static int launch_bsvd(const BatchArgs<T> &a, int grid, hipStream_t s)
 {
-  const size_t lds = LDS ? (size_t)a.lds_bytes : 0;
+  const size_t lds = launch_lds<T, CPLX, E, W, GRP, LDS>(a);
   if (lds > 64 * 1024)
     VJ_HIP(hipFuncSetAttribute((const void *)k_bsvd<T, CPLX, E, W, GRP, LDS, LG>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1474,9 +1702,9 @@ static int dispatch_E(const BatchArgs<T> &a, int max_m, int grid_cap, hipStream_
   int cus = 0, blocks = 1, dev = 0;
   VJ_HIP(hipGetDevice(&dev));
   VJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const size_t lds = LDS ? (size_t)a.lds_bytes : 0;
 #define VJ_BS_CASE(EE)                                                         \
   {                                                                            \
+    const size_t lds = launch_lds<T, CPLX, EE, W, GRP, LDS>(a);                \
     if (int rc = occupancy<T, CPLX, EE, W, GRP, LDS, LG>(&blocks, lds))        \
       return rc;                                                               \
     long long g = (long long)cus * (blocks > 0 ? blocks : 1);                  \
