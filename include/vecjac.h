@@ -299,6 +299,14 @@ vjh_int vjh_bplan_order(void *plan, int id);
  * model < 0 returns the current model; > 2 returns -2.  The results do not
  * depend on it; the autotuner picks it (step 3). */
 vjh_int vjh_bplan_queue(void *plan, int model);
+/* Preset V (the batched form of iwork[0] != 0, dvjsvd.c:86-96,143-157): with
+ * on = 1 every run takes each V_b (n_b x n_b at offV[b]) as the starting
+ * matrix instead of forming I, tracks its max |.| and rescales it as the
+ * plain drivers do, and a matrix with a non-finite V fails with -92
+ * (complex -104), a zero V with -13 (-17).  Each matrix equals the plain
+ * driver run with iwork[0] = 1 on the same table.  Runs with a preset V take
+ * the step-by-step form (no block rounds).  on < 0 returns the setting. */
+vjh_int vjh_bplan_vpreset(void *plan, int on);
 /* Autotuner of a plan, on copies of the input G (Gi) made into the scratch
  * buffer Gw (Gwi), each launch timed on its own with HIP events on `stream`:
  * (1) for the split row count and half of it, each launch at 1, 2, 4 and 8

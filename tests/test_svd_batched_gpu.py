@@ -27,10 +27,12 @@ def _batch(nb, seed, nmax=48, dtype=np.float64):
 
 
 def _run(kind, m, npad, G, tot, waves=0, Gi=None, split=None, budget=None, wv=None, lanes=None, order=None,
-         queue=None):
+         queue=None, V0=None, V0i=None, swp=999):
     import torch
     dt = torch.float64 if kind in "dz" else torch.float32
-    plan = BT.BatchedSVD(m, npad, kind=kind, waves=waves)
+    plan = BT.BatchedSVD(m, npad, kind=kind, waves=waves, swp=swp)
+    if V0 is not None:
+        assert plan.vpreset(1)
     if order is not None:
         assert plan.order(order) == order
     if queue is not None:
@@ -39,8 +41,10 @@ def _run(kind, m, npad, G, tot, waves=0, Gi=None, split=None, budget=None, wv=No
         plan.set(split if split is not None else (64 if kind in "zc" else 128), budget, wv, lanes)
     Gd = torch.from_numpy(G.copy()).cuda()
     Gid = torch.from_numpy(Gi.copy()).cuda() if Gi is not None else None
-    V = torch.zeros(tot[1], dtype=dt, device="cuda")
-    Vi = torch.zeros(tot[1], dtype=dt, device="cuda") if Gi is not None else None
+    V = torch.zeros(tot[1], dtype=dt, device="cuda") if V0 is None else torch.from_numpy(V0.copy()).cuda()
+    Vi = None
+    if Gi is not None:
+        Vi = torch.zeros(tot[1], dtype=dt, device="cuda") if V0i is None else torch.from_numpy(V0i.copy()).cuda()
     eS = torch.zeros(tot[2], dtype=dt, device="cuda")
     fS = torch.zeros(tot[2], dtype=dt, device="cuda")
     info = torch.zeros(len(m), dtype=torch.int64, device="cuda")
@@ -387,3 +391,83 @@ def test_queue_models_same_bits():
     for r in runs[1:]:
         for f in ("G", "V", "eS", "fS", "info", "tr"):
             assert np.array_equal(r[f], runs[0][f], equal_nan=True), f
+
+
+@pytest.mark.parametrize("kind", ["d", "z"])
+def test_preset_v_vs_plain_bitwise(kind):
+    """A preset V (vjh_bplan_vpreset, the batched iwork[0] = 1): each member
+    against the plain driver run with vpreset on the same G, V and table,
+    bit for bit: long and short members, both sweep-limited first passes'
+    outputs resumed and random orthogonal starting matrices scaled by 2^300
+    (which takes the initial and tracked V rescalings); a zero V fails with
+    -13 (-17), an infinite entry with -92 (-104)."""
+    import torch
+    m = np.array([200, 96, 64, 300, 48], dtype=np.int64)
+    n = np.array([64, 40, 32, 96, 48], dtype=np.int64)
+    _, npad, offG, offV, offS, tot = BT.pack_shapes(m, n)
+    G = BT.fill_gaussian(m, npad, n, offG, 611)
+    Gi = BT.fill_gaussian(m, npad, n, offG, 612) if kind == "z" else None
+    g = W.rng(613)
+    V0 = np.zeros(tot[1])
+    V0i = np.zeros(tot[1]) if kind == "z" else None
+    for b in range(len(m)):
+        nn = int(npad[b])
+        Q = W.haar_unitary(nn, nn, g, kind == "z") * 2.0 ** 300
+        V0[offV[b]: offV[b] + nn * nn] = Q.real.T.reshape(-1)
+        if kind == "z":
+            V0i[offV[b]: offV[b] + nn * nn] = Q.imag.T.reshape(-1)
+    V0[offV[2]: offV[2] + int(npad[2]) ** 2] = 0.0          # member 2: zero V
+    if kind == "z":
+        V0i[offV[2]: offV[2] + int(npad[2]) ** 2] = 0.0
+    V0[offV[4] + 7] = np.inf                                # member 4: infinite entry
+    r = _run(kind, m, npad, G, tot, Gi=Gi, V0=V0, V0i=V0i)
+    assert r["info"][2] == (-17 if kind == "z" else -13)
+    assert r["info"][4] == (-104 if kind == "z" else -92)
+    for b in (0, 1, 3):
+        mm, nn = int(m[b]), int(npad[b])
+        js, stp = S.pjs(6, nn)
+        sl, sv = slice(offG[b], offG[b] + mm * nn), slice(offV[b], offV[b] + nn * nn)
+        Gd = torch.from_numpy(G[sl].reshape(nn, mm).copy()).cuda()
+        Vd = torch.from_numpy(V0[sv].reshape(nn, nn).copy()).cuda()
+        Gid = torch.from_numpy(Gi[sl].reshape(nn, mm).copy()).cuda() if kind == "z" else None
+        Vid = torch.from_numpy(V0i[sv].reshape(nn, nn).copy()).cuda() if kind == "z" else None
+        eS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        fS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        jsd = torch.from_numpy(js.reshape(-1).view(np.int32)).cuda()
+        print(f"plain {kind} vpreset m={mm} n={nn}", flush=True)
+        rc, _ = S.device_vjsvd(kind, Gd, Vd, eS, fS, jsd, stp, Gi=Gid, Vi=Vid, vpreset=True)
+        assert rc == int(r["info"][b]) and rc > 0, (b, rc, r["info"][b])
+        assert np.array_equal(Gd.cpu().numpy().reshape(-1), r["G"][sl], equal_nan=True), b
+        assert np.array_equal(Vd.cpu().numpy().reshape(-1), r["V"][sv]), b
+        if kind == "z":
+            assert np.array_equal(Gid.cpu().numpy().reshape(-1), r["Gi"][sl], equal_nan=True), b
+            assert np.array_equal(Vid.cpu().numpy().reshape(-1), r["Vi"][sv]), b
+        assert np.array_equal(eS.cpu().numpy(), r["eS"][offS[b]: offS[b] + nn]), b
+        assert np.array_equal(fS.cpu().numpy(), r["fS"][offS[b]: offS[b] + nn]), b
+
+
+def test_preset_v_resume_equals_plain_resume():
+    """A sweep-limited batched run (2 sweeps) resumed with its own working G
+    and V as the preset equals the plain driver resumed the same way, bit
+    for bit, for every member."""
+    import torch
+    m, n, npad, offG, offV, offS, tot, G = _batch(12, 717, nmax=80)
+    r1 = _run("d", m, npad, G, tot, swp=2)
+    assert np.all(r1["info"] == 2)
+    r2 = _run("d", m, npad, r1["G"], tot, V0=r1["V"])
+    for b in range(len(m)):
+        mm, nn = int(m[b]), int(npad[b])
+        js, stp = S.pjs(6, nn)
+        sl, sv = slice(offG[b], offG[b] + mm * nn), slice(offV[b], offV[b] + nn * nn)
+        Gd = torch.from_numpy(G[sl].reshape(nn, mm).copy()).cuda()
+        Vd = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
+        eS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        fS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        jsd = torch.from_numpy(js.reshape(-1).view(np.int32)).cuda()
+        rc1, _ = S.device_vjsvd("d", Gd, Vd, eS, fS, jsd, stp, swp=2)
+        assert rc1 == 2
+        rc2, _ = S.device_vjsvd("d", Gd, Vd, eS, fS, jsd, stp, vpreset=True)
+        assert rc2 == int(r2["info"][b]) and rc2 >= 0, (b, rc2, r2["info"][b])
+        assert np.array_equal(Gd.cpu().numpy().reshape(-1), r2["G"][sl], equal_nan=True), b
+        assert np.array_equal(Vd.cpu().numpy().reshape(-1), r2["V"][sv]), b
+        assert np.array_equal(eS.cpu().numpy(), r2["eS"][offS[b]: offS[b] + nn]), b

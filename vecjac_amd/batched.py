@@ -40,6 +40,8 @@ def _declare(L):
     L.vjh_bplan_order.argtypes = [C.c_void_p, C.c_int]
     L.vjh_bplan_queue.restype = C.c_int64
     L.vjh_bplan_queue.argtypes = [C.c_void_p, C.c_int]
+    L.vjh_bplan_vpreset.restype = C.c_int64
+    L.vjh_bplan_vpreset.argtypes = [C.c_void_p, C.c_int]
     L.vjh_bplan_set.restype = C.c_int64
     L.vjh_bplan_set.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p]
     L.vjh_bplan_config.restype = C.c_int64
@@ -172,6 +174,11 @@ class BatchedSVD:
         if rc < 0:
             raise ValueError(f"vjh_bplan_queue: {_lib.last_error()}")
         return int(rc)
+
+    def vpreset(self, on: int = -1) -> bool:
+        """Whether runs take V as the starting matrices (vjh_bplan_vpreset);
+        on = 1 / 0 sets it, -1 only returns it."""
+        return bool(_lib.lib().vjh_bplan_vpreset(self.plan, int(on)))
 
     def close(self):
         if getattr(self, "plan", None):

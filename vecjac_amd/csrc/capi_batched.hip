@@ -42,6 +42,7 @@ struct BPlan {
   unsigned swp = 999u;
   int ord = 6;             // ordering of the pairs (pjs id)
   int queue = 0;           // cost model of the processing order (queue_cost)
+  int vpreset = 0;         // V holds the starting matrices on entry
   int dev = 0;
   std::vector<long long> hm, hn; // the shapes, for re-partitioning
   long long *d_meta = nullptr; // m, n, offG, offV, offS (nb each)
@@ -161,6 +162,7 @@ int run_plan(const BPlan *P, void *Gr, void *Gi, void *Vr, void *Vi, void *eS, v
   a.xfer = P->d_xfer;
   a.swp = P->swp;
   a.ord = P->ord;
+  a.vpreset = P->vpreset;
   // a fill kernel, not hipMemsetAsync, so that the run can be graph-captured
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, P->d_queue);
   // the long columns first (the costly matrices start first), then the
@@ -496,6 +498,16 @@ vjh_int vjh_bplan_queue(void *plan, int model)
   if (int rc = partition(P))
     return rc;
   return model;
+}
+
+vjh_int vjh_bplan_vpreset(void *plan, int on)
+{
+  BPlan *P = (BPlan *)plan;
+  if (!P)
+    return set_error(-1, "null plan");
+  if (on >= 0)
+    P->vpreset = on ? 1 : 0;
+  return P->vpreset;
 }
 
 vjh_int vjh_bplan_order(void *plan, int id)

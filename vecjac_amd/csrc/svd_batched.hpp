@@ -21,6 +21,7 @@ template <typename T> struct BatchArgs {
   int lds_bytes;                  // LDS-resident form: bytes of G and V per workgroup (0: global)
   int group_lg;                   // lane-group form: 2^group_lg lanes per pair (4, 3 or 2)
   int ord;                        // ordering of the pairs: pjs id 4 (modulus) or 6 (block modulus)
+  int vpreset;                    // V holds the starting matrices (else V := I)
 };
 
 // Shapes of the lane-group form that are built: 16 lanes per pair with any

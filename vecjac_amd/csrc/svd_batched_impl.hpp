@@ -163,11 +163,12 @@ __device__ __forceinline__ void store_vpair(T *Vr, T *Vi, const int n, const int
 }
 
 // The V part of pair_apply for V held in registers: the same swap or
-// rotation, with the same fused multiply-adds (V := I at the start, so no
-// max-|.| tracking).
+// rotation, with the same fused multiply-adds.  Returns the lane's max
+// |updated element| of a rotation (the tracking of a preset V, as v_apply).
 template <typename T, bool CPLX>
-__device__ __forceinline__ void v_update(const PairDec<T> &D, T (&vx)[2], T (&vy)[2], T (&vxi)[2], T (&vyi)[2])
+__device__ __forceinline__ T v_update(const PairDec<T> &D, T (&vx)[2], T (&vy)[2], T (&vxi)[2], T (&vyi)[2])
 {
+  T tV = (T)0;
   const int code = D.code;
   const T c = D.c, at = D.at, sat = D.sat;
   bool swap = (code == -1) || (code == -3);
@@ -187,6 +188,7 @@ __device__ __forceinline__ void v_update(const PairDec<T> &D, T (&vx)[2], T (&vy
           if (!c1) { a = a * c; b = b * c; }
           vx[j] = a;
           vy[j] = b;
+          tV = max1p(tV, max1p(vabs(a), vabs(b)));
           if constexpr (CPLX) {
             const T x_i = vxi[j], y_i = vyi[j];
             T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
@@ -194,6 +196,7 @@ __device__ __forceinline__ void v_update(const PairDec<T> &D, T (&vx)[2], T (&vy
             if (!c1) { a2 = a2 * c; b2 = b2 * c; }
             vxi[j] = a2;
             vyi[j] = b2;
+            tV = max1p(tV, max1p(vabs(a2), vabs(b2)));
           }
         }
         else if constexpr (CPLX) {
@@ -203,6 +206,7 @@ __device__ __forceinline__ void v_update(const PairDec<T> &D, T (&vx)[2], T (&vy
           T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
           T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
           if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
+          tV = xmax(tV, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
           if (perm) { vx[j] = br; vxi[j] = bi; vy[j] = ar; vyi[j] = ai; }
           else { vx[j] = ar; vxi[j] = ai; vy[j] = br; vyi[j] = bi; }
         }
@@ -216,6 +220,7 @@ __device__ __forceinline__ void v_update(const PairDec<T> &D, T (&vx)[2], T (&vy
       if constexpr (CPLX) { t = vxi[j]; vxi[j] = vyi[j]; vyi[j] = t; }
     }
   }
+  return tV;
 }
 
 // Pairs a wave works on together (their reductions share the cross-lane
@@ -266,8 +271,8 @@ template <typename T> __device__ __forceinline__ PairDec<T> lane_dec(const PairD
 template <typename T, bool CPLX, int E, int W, int GP, bool PF>
 __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n, const int nE,
                                           const int ord, const unsigned st, const int wave, const int lane,
-                                          const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr, int *mod,
-                                          const int gstep, unsigned long long &wev, unsigned long long &wxf)
+                                          const Step<T> &S, T &wG, T &wV, unsigned long long &wcnt, int &werr,
+                                          int *mod, const int gstep, unsigned long long &wev, unsigned long long &wxf)
 {
   typedef typename Acc<T>::A A;
   constexpr int B = 64;
@@ -404,14 +409,18 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
           mod[p[r]] = mod[q[r]] = gstep;
         wxf += 2ull * PL * (unsigned long long)m;
       }
+      T tV = (T)0;
       if (S.sV || (!D.err && (D.code != 1) && (D.code != 3))) {
         if (S.sV)
           scale_vpair<T>(S.sV, vx[r], vy[r], vxi[r], vyi[r]);
         if (!D.err)
-          v_update<T, CPLX>(D, vx[r], vy[r], vxi[r], vyi[r]);
+          tV = v_update<T, CPLX>(D, vx[r], vy[r], vxi[r], vyi[r]);
         store_vpair<T, CPLX>(Vr, Vi, n, (int)p[r], (int)q[r], lane, vx[r], vy[r], vxi[r], vyi[r]);
         wxf += 2ull * PL * (unsigned long long)n;
       }
+      // a preset V is tracked as djrot_ tracks it (an overflowed update fails)
+      if (S.vtrack && !err && (__ballot(!isfinite_(tV)) != 0ull))
+        err = CPLX ? -638 : -588;
       if (err)
         werr = err;
       else {
@@ -420,6 +429,8 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
           eS[q[r]] = eSq; fS[q[r]] = fSq;
         }
         wG = xmax(wG, o.tG);
+        if (S.vtrack)
+          wV = xmax(wV, tV);
         wcnt += (unsigned long long)o.trans;
       }
     }
@@ -1133,7 +1144,7 @@ template <typename T, bool CPLX, int E> struct GroupShape {
 template <typename T, bool CPLX, int E, int W, bool PF, int LG>
 __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                            const int ord, const unsigned st, const int wave, const int lane,
-                                           const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr,
+                                           const Step<T> &S, T &wG, T &wV, unsigned long long &wcnt, int &werr,
                                            unsigned long long &wev, unsigned long long *wxf)
 {
   typedef typename Acc<T>::A A;
@@ -1238,15 +1249,15 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
     // destination columns exchanged: a swap (-1), or a rotation or a unit
     // swap with the permutation (-2)
     const bool dsw = ok && ((code == -1) || ((code == -2) && (!at0 || c1)));
-    T tG = (T)0;
+    T tG = (T)0, tV = (T)0;
     if (rotr) {
       tG = grp_rot_real<T, CPLX, E>(xr, yr, xi, yi, D.c, D.at, c1, nE);
-      grp_rot_real<T, CPLX, E>(vx, vy, vxi, vyi, D.c, D.at, c1, nEV);
+      tV = grp_rot_real<T, CPLX, E>(vx, vy, vxi, vyi, D.c, D.at, c1, nEV);
     }
     if constexpr (CPLX) {
       if (rotc) {
         tG = grp_rot_cplx<T, E>(xr, yr, xi, yi, D.c, D.at, D.sat, c1, nE);
-        grp_rot_cplx<T, E>(vx, vy, vxi, vyi, D.c, D.at, D.sat, c1, nEV);
+        tV = grp_rot_cplx<T, E>(vx, vy, vxi, vyi, D.c, D.at, D.sat, c1, nEV);
       }
     }
     if (gs) {
@@ -1265,6 +1276,9 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
     int err = D.err;
     if (!err && ((__ballot(!isfinite_(tG)) & gmask) != 0ull))
       err = CPLX ? -630 : -580;
+    // a preset V is tracked as djrot_ tracks it (an overflowed update fails)
+    if (S.vtrack && !err && ((__ballot(!isfinite_(tV)) & gmask) != 0ull))
+      err = CPLX ? -638 : -588;
     const unsigned dp = dsw ? q : p, dq = dsw ? p : q;
     const bool stG = valid && !err && ((code != 1) || S.sG);
     const bool stV = valid && (S.sV || (ok && (code != 1) && (code != 3)));
@@ -1287,8 +1301,11 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
     const unsigned long long eb = __ballot(valid && (err != 0));
     if (eb)
       werr = __builtin_amdgcn_readlane(err, (int)__builtin_ctzll(eb));
-    if (valid && !err)
+    if (valid && !err) {
       wG = xmax(wG, tG);
+      if (S.vtrack)
+        wV = xmax(wV, tV);
+    }
     wcnt += (unsigned long long)__popcll(__ballot(valid && !err && (t == 0) && D.trans));
     if constexpr (PF) {
       p = pn;
@@ -1313,18 +1330,18 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
 template <typename T, bool CPLX, int E, int W, int LG>
 __device__ __forceinline__ void group_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                                const int ord, const unsigned st, const int wave, const int lane,
-                                               const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr,
+                                               const Step<T> &S, T &wG, T &wV, unsigned long long &wcnt, int &werr,
                                                unsigned long long &wev, unsigned long long *wxf)
 {
   constexpr int E64 = 64 >> LG; // chunks of 2^LG rows for 64 rows
   if ((E < 2) || CPLX || (m <= 64)) {
     group_step<T, CPLX, E64, W, GroupShape<T, CPLX, E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, ord, st, wave, lane, S,
-                                                                  wG, wcnt, werr, wev, wxf);
+                                                                  wG, wV, wcnt, werr, wev, wxf);
     return;
   }
   if constexpr ((E >= 2) && !CPLX && (2 * E64 <= 16))
     group_step<T, CPLX, 2 * E64, W, GroupShape<T, CPLX, 2 * E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, ord, st, wave,
-                                                                          lane, S, wG, wcnt, werr, wev, wxf);
+                                                                          lane, S, wG, wV, wcnt, werr, wev, wxf);
 }
 
 // Longer columns: a pair per 64 lanes, two pairs per wave, with the register
@@ -1332,13 +1349,14 @@ __device__ __forceinline__ void group_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS
 template <typename T, bool CPLX, int E, int W>
 __device__ __forceinline__ void wave_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                               const int ord, const unsigned st, const int wave, const int lane,
-                                              const Step<T> &S, T &wG, unsigned long long &wcnt, int &werr, int *mod,
-                                              const int gstep, unsigned long long &wev, unsigned long long &wxf)
+                                              const Step<T> &S, T &wG, T &wV, unsigned long long &wcnt, int &werr,
+                                              int *mod, const int gstep, unsigned long long &wev,
+                                              unsigned long long &wxf)
 {
   const int nE = (m + 63) >> 6;
 #define VJ_WS(EE)                                                                                                     \
   wave_step<T, CPLX, EE, W, StepShape<T, CPLX, EE>::GP, StepShape<T, CPLX, EE>::PF>(                                   \
-      Gr, Gi, Vr, Vi, eS, fS, m, n, nE, ord, st, wave, lane, S, wG, wcnt, werr, mod, gstep, wev, wxf)
+      Gr, Gi, Vr, Vi, eS, fS, m, n, nE, ord, st, wave, lane, S, wG, wV, wcnt, werr, mod, gstep, wev, wxf)
   if ((E < 4) || (nE <= 2)) {
     constexpr int E2 = (E >= 2) ? 2 : E;
     VJ_WS(E2);
@@ -1466,7 +1484,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     // by |g_ij| <= |G|_F <= sqrt(m n) max |g| < M/2, i.e. below 2^(ROT_EXP_R
     // + 1) with a margin of at least 2^-10 relative (m <= 1024), which the
     // rounding of the rotations cannot close; V stays orthogonal.
-    const bool blk = !GRP && !CPLX && (a.ord == 6);
+    const bool blk = !GRP && !CPLX && (a.ord == 6) && !a.vpreset;
 
     // ---- setup (dvjsvd.c:78-184): max |G|, V := I, eS/fS, initial scale
     T mxv = (T)0;
@@ -1492,16 +1510,48 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     for (int w = 1; w < W; ++w)
       MG = xmax(MG, (T)s_max[0][w][0]);
     __syncthreads();
+    // a preset V (dvjsvd.c:89-96): its max |.| (copied into LDS with it)
+    T MV0 = (T)1;
+    if (a.vpreset) {
+      T mv = (T)0;
+      for (int t = threadIdx.x; t < n * n; t += 64 * W) {
+        const T v = gVr[t];
+        mv = xmax(xmin(vabs(v), inf_<T>()), mv);
+        if constexpr (LDS)
+          Vr[t] = v;
+        if constexpr (CPLX) {
+          const T vi = gVi[t];
+          mv = xmax(xmin(vabs(vi), inf_<T>()), mv);
+          if constexpr (LDS)
+            Vi[t] = vi;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1)
+        mv = xmax(mv, __shfl_xor(mv, o, 64));
+      if (lane == 0)
+        s_max[1][wave][0] = (double)mv;
+      __syncthreads();
+      MV0 = (T)s_max[1][0][0];
+      for (int w = 1; w < W; ++w)
+        MV0 = xmax(MV0, (T)s_max[1][w][0]);
+      __syncthreads();
+    }
     int status = 0;
     if (!(MG <= Lim<T>::MAX))
       status = CPLX ? -92 : -80;
+    else if (a.vpreset && !(MV0 <= Lim<T>::MAX))
+      status = CPLX ? -104 : -92;
+    else if (a.vpreset && !(MV0 > (T)0))
+      status = CPLX ? -17 : -13;
     if (!status) {
-      for (int t = threadIdx.x; t < n * n; t += 64 * W) {
-        const int j = t / n, i = t - j * n;
-        Vr[t] = (i == j) ? (T)1 : (T)0;
-        if constexpr (CPLX)
-          Vi[t] = (T)0;
-      }
+      if (!a.vpreset)
+        for (int t = threadIdx.x; t < n * n; t += 64 * W) {
+          const int j = t / n, i = t - j * n;
+          Vr[t] = (i == j) ? (T)1 : (T)0;
+          if constexpr (CPLX)
+            Vi[t] = (T)0;
+        }
       for (int j = threadIdx.x; j < n; j += 64 * W) {
         s_eS[j] = -inf_<T>();
         s_fS[j] = (T)1;
@@ -1514,7 +1564,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     unsigned long long xset = (unsigned long long)m * (unsigned long long)n + (norms ? (unsigned long long)n * n : 0ull);
     unsigned sw = 0;
     unsigned long long total = 0;
-    int sT = 0, NRM_EXP = 0;
+    int sT = 0, NRM_EXP = 0, sVt = 0; // sVt: the shifts of a preset V
     T MV = (T)1;
     if (!status && (MG > (T)0)) {
       const T M_m = CPLX ? (Lim<T>::MAX / ((T)(m << 2) * (T)1.41421356237309504880)) : (Lim<T>::MAX / (T)(m << 1));
@@ -1531,13 +1581,27 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
         }
         xset += 2ull * (unsigned long long)m * (unsigned long long)n;
       }
+      if (a.vpreset) {
+        // its initial scaling (dvjsvd.c:143-156): max |V| to 2^rot
+        const int eV = frexp_exp(MV0) - 1;
+        sVt = rot - eV;
+        MV = ldexp_(MV0, sVt);
+        if (sVt) {
+          for (int t = threadIdx.x; t < n * n; t += 64 * W) {
+            Vr[t] = ldexp_(Vr[t], sVt);
+            if constexpr (CPLX)
+              Vi[t] = ldexp_(Vi[t], sVt);
+          }
+          xset += 2ull * (unsigned long long)n * (unsigned long long)n;
+        }
+      }
       __syncthreads();
       Step<T> S;
       S.tol = sqrt_((T)m) * (Lim<T>::EPS / (T)2);
       S.gst = ldexp_(S.tol, (int)Lim<T>::FIN_EXP);
       S.sG = 0;
       S.sV = 0;
-      S.vtrack = 0;
+      S.vtrack = a.vpreset ? 1 : 0;
       S.sec = 0;
       int par = 0, gstep = 0;
       for (sw = 0; sw < a.swp;) {
@@ -1548,8 +1612,8 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           unsigned long long wcnt = 0;
           int werr = 0;
           if constexpr (GRP)
-            group_dispatch<T, CPLX, E, W, LG>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, a.ord, st, wave, lane, S, wG, wcnt,
-                                              werr, wev, LDS ? nullptr : &wxf);
+            group_dispatch<T, CPLX, E, W, LG>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, a.ord, st, wave, lane, S, wG, wV,
+                                              wcnt, werr, wev, LDS ? nullptr : &wxf);
           else {
             if constexpr (!CPLX) {
               if (blk) {
@@ -1560,10 +1624,12 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
               }
             }
             if (!blk)
-              wave_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, a.ord, st, wave, lane, S, wG, wcnt, werr,
-                                           s_mod, gstep, wev, wxf);
+              wave_dispatch<T, CPLX, E, W>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, a.ord, st, wave, lane, S, wG, wV, wcnt,
+                                           werr, s_mod, gstep, wev, wxf);
           }
           wG = red_max<T, 64>(wG, nullptr);
+          if (S.vtrack)
+            wV = red_max<T, 64>(wV, nullptr);
           if (lane == 0) {
             s_max[par][wave][0] = (double)wG;
             s_max[par][wave][1] = (double)wV;
@@ -1602,6 +1668,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
             if (rot - eV < 0) {
               S.sV = rot - eV;
               MV = ldexp_(MV, S.sV);
+              sVt += S.sV;
             }
           }
           __threadfence_block();
@@ -1614,7 +1681,8 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
         ++sw;
       }
       __syncthreads();
-      // ---- epilogue (dvjsvd.c:604-631); V is not rescaled when it starts as I
+      // ---- epilogue (dvjsvd.c:604-631); V is rescaled only when preset (V
+      // := I is never rescaled: it stays orthogonal)
       if (!status && (sw < a.swp)) {
         for (long long t = threadIdx.x; t < (long long)m * n; t += 64 * W) {
           const int j = (int)(t / m);
@@ -1622,6 +1690,15 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           Gr[t] = scalef(Gr[t] / f_, s_);
           if constexpr (CPLX)
             Gi[t] = scalef(Gi[t] / f_, s_);
+        }
+        if (sVt) {
+          const T sv = (T)(-sVt);
+          for (int t = threadIdx.x; t < n * n; t += 64 * W) {
+            Vr[t] = scalef(Vr[t], sv);
+            if constexpr (CPLX)
+              Vi[t] = scalef(Vi[t], sv);
+          }
+          xset += 2ull * (unsigned long long)n * (unsigned long long)n;
         }
         __syncthreads();
         for (int j = threadIdx.x; j < n; j += 64 * W)
