@@ -163,9 +163,10 @@ __device__ __forceinline__ void store_vpair(T *Vr, T *Vi, const int n, const int
 }
 
 // The V part of pair_apply for V held in registers: the same swap or
-// rotation, with the same fused multiply-adds.  Returns the lane's max
-// |updated element| of a rotation (the tracking of a preset V, as v_apply).
-template <typename T, bool CPLX>
+// rotation, with the same fused multiply-adds.  With TRK, returns the lane's
+// max |updated element| of a rotation (the tracking of a preset V, as
+// v_apply); otherwise 0, and the maxima are not formed.
+template <typename T, bool CPLX, bool TRK = false>
 __device__ __forceinline__ T v_update(const PairDec<T> &D, T (&vx)[2], T (&vy)[2], T (&vxi)[2], T (&vyi)[2])
 {
   T tV = (T)0;
@@ -188,7 +189,8 @@ __device__ __forceinline__ T v_update(const PairDec<T> &D, T (&vx)[2], T (&vy)[2
           if (!c1) { a = a * c; b = b * c; }
           vx[j] = a;
           vy[j] = b;
-          tV = max1p(tV, max1p(vabs(a), vabs(b)));
+          if constexpr (TRK)
+            tV = max1p(tV, max1p(vabs(a), vabs(b)));
           if constexpr (CPLX) {
             const T x_i = vxi[j], y_i = vyi[j];
             T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
@@ -196,7 +198,8 @@ __device__ __forceinline__ T v_update(const PairDec<T> &D, T (&vx)[2], T (&vy)[2
             if (!c1) { a2 = a2 * c; b2 = b2 * c; }
             vxi[j] = a2;
             vyi[j] = b2;
-            tV = max1p(tV, max1p(vabs(a2), vabs(b2)));
+            if constexpr (TRK)
+              tV = max1p(tV, max1p(vabs(a2), vabs(b2)));
           }
         }
         else if constexpr (CPLX) {
@@ -206,7 +209,8 @@ __device__ __forceinline__ T v_update(const PairDec<T> &D, T (&vx)[2], T (&vy)[2
           T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
           T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
           if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
-          tV = xmax(tV, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
+          if constexpr (TRK)
+            tV = xmax(tV, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
           if (perm) { vx[j] = br; vxi[j] = bi; vy[j] = ar; vyi[j] = ai; }
           else { vx[j] = ar; vxi[j] = ai; vy[j] = br; vyi[j] = bi; }
         }
@@ -413,8 +417,12 @@ __device__ __forceinline__ void wave_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *
       if (S.sV || (!D.err && (D.code != 1) && (D.code != 3))) {
         if (S.sV)
           scale_vpair<T>(S.sV, vx[r], vy[r], vxi[r], vyi[r]);
-        if (!D.err)
-          tV = v_update<T, CPLX>(D, vx[r], vy[r], vxi[r], vyi[r]);
+        if (!D.err) {
+          if (S.vtrack)
+            tV = v_update<T, CPLX, true>(D, vx[r], vy[r], vxi[r], vyi[r]);
+          else
+            v_update<T, CPLX>(D, vx[r], vy[r], vxi[r], vyi[r]);
+        }
         store_vpair<T, CPLX>(Vr, Vi, n, (int)p[r], (int)q[r], lane, vx[r], vy[r], vxi[r], vyi[r]);
         wxf += 2ull * PL * (unsigned long long)n;
       }
@@ -994,11 +1002,10 @@ __device__ __forceinline__ void store_cols(T *Ar, T *Ai, const int rows, const i
 }
 
 // Real rotation (x, y) := ((x + t y) c, (y - t x) c) of E chunks, the
-// scaling skipped where c == 1 (the product by one is exact); returns the
-// lane's max |updated element|.
+// scaling skipped where c == 1 (the product by one is exact).
 template <typename T, bool CPLX, int E>
-__device__ __forceinline__ T grp_rot_real(T (&x)[E], T (&y)[E], T (&xi)[E], T (&yi)[E], const T c, const T at,
-                                          const bool c1, const int nE)
+__device__ __forceinline__ void grp_rot_real(T (&x)[E], T (&y)[E], T (&xi)[E], T (&yi)[E], const T c, const T at,
+                                             const bool c1, const int nE)
 {
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -1026,6 +1033,14 @@ __device__ __forceinline__ T grp_rot_real(T (&x)[E], T (&y)[E], T (&xi)[E], T (&
       }
     }
   }
+}
+
+// The lane's max |element| of a pair after grp_rot_real (one product per
+// element, so the IEEE maximum).
+template <typename T, bool CPLX, int E>
+__device__ __forceinline__ T grp_max_real(const T (&x)[E], const T (&y)[E], const T (&xi)[E], const T (&yi)[E],
+                                          const int nE)
+{
   T tG = (T)0;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -1040,11 +1055,10 @@ __device__ __forceinline__ T grp_rot_real(T (&x)[E], T (&y)[E], T (&xi)[E], T (&
 
 // Complex rotation with a complex tangent at + i sat (zjrot_, zjrot.c:36-60).
 template <typename T, int E>
-__device__ __forceinline__ T grp_rot_cplx(T (&x)[E], T (&y)[E], T (&xi)[E], T (&yi)[E], const T c, const T at,
-                                          const T sat, const bool c1, const int nE)
+__device__ __forceinline__ void grp_rot_cplx(T (&x)[E], T (&y)[E], T (&xi)[E], T (&yi)[E], const T c, const T at,
+                                             const T sat, const bool c1, const int nE)
 {
   const T cna = -at;
-  T tG = (T)0;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     if (j >= nE)
@@ -1055,8 +1069,22 @@ __device__ __forceinline__ T grp_rot_cplx(T (&x)[E], T (&y)[E], T (&xi)[E], T (&
     T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
     T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
     if (!c1) { ar = ar * c; ai = ai * c; br = br * c; bi = bi * c; }
-    tG = xmax(tG, xmax(xmax(vabs(ar), vabs(ai)), xmax(vabs(br), vabs(bi))));
     x[j] = ar; xi[j] = ai; y[j] = br; yi[j] = bi;
+  }
+}
+
+// The lane's max |element| of a pair after grp_rot_cplx (two products
+// summed per element: the reference's ordered maximum, in its order).
+template <typename T, int E>
+__device__ __forceinline__ T grp_max_cplx(const T (&x)[E], const T (&y)[E], const T (&xi)[E], const T (&yi)[E],
+                                          const int nE)
+{
+  T tG = (T)0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    if (j >= nE)
+      break;
+    tG = xmax(tG, xmax(xmax(vabs(x[j]), vabs(xi[j])), xmax(vabs(y[j]), vabs(yi[j]))));
   }
   return tG;
 }
@@ -1251,13 +1279,19 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
     const bool dsw = ok && ((code == -1) || ((code == -2) && (!at0 || c1)));
     T tG = (T)0, tV = (T)0;
     if (rotr) {
-      tG = grp_rot_real<T, CPLX, E>(xr, yr, xi, yi, D.c, D.at, c1, nE);
-      tV = grp_rot_real<T, CPLX, E>(vx, vy, vxi, vyi, D.c, D.at, c1, nEV);
+      grp_rot_real<T, CPLX, E>(xr, yr, xi, yi, D.c, D.at, c1, nE);
+      tG = grp_max_real<T, CPLX, E>(xr, yr, xi, yi, nE);
+      grp_rot_real<T, CPLX, E>(vx, vy, vxi, vyi, D.c, D.at, c1, nEV);
+      if (S.vtrack)
+        tV = grp_max_real<T, CPLX, E>(vx, vy, vxi, vyi, nEV);
     }
     if constexpr (CPLX) {
       if (rotc) {
-        tG = grp_rot_cplx<T, E>(xr, yr, xi, yi, D.c, D.at, D.sat, c1, nE);
-        tV = grp_rot_cplx<T, E>(vx, vy, vxi, vyi, D.c, D.at, D.sat, c1, nEV);
+        grp_rot_cplx<T, E>(xr, yr, xi, yi, D.c, D.at, D.sat, c1, nE);
+        tG = grp_max_cplx<T, E>(xr, yr, xi, yi, nE);
+        grp_rot_cplx<T, E>(vx, vy, vxi, vyi, D.c, D.at, D.sat, c1, nEV);
+        if (S.vtrack)
+          tV = grp_max_cplx<T, E>(vx, vy, vxi, vyi, nEV);
       }
     }
     if (gs) {
