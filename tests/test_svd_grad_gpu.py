@@ -124,20 +124,24 @@ def test_failed_member_raises():
     assert info[4] == -80 and np.all(np.delete(info, 4) >= 0)
 
 
-@pytest.mark.parametrize("dt,m,n", [("d", 64, 32), ("d", 50, 22), ("d", 33, 17), ("d", 600, 64), ("d", 96, 96),
-                                    ("d", 7, 2), ("s", 64, 32), ("s", 130, 40),
-                                    # wider than the LDS form: J/K in a global workspace
-                                    ("d", 300, 128), ("d", 112, 112), ("s", 256, 128)])
-def test_backward_kernel_vs_closed_form(dt, m, n):
+@pytest.mark.parametrize("dt,m,n,nb", [("d", 64, 32, 5), ("d", 50, 22, 5), ("d", 33, 17, 5), ("d", 600, 64, 5),
+                                       ("d", 96, 96, 5), ("d", 7, 2, 5), ("s", 64, 32, 5), ("s", 130, 40, 5),
+                                       # staged form, more matrices than workgroups (each workgroup
+                                       # computes several, the next one's operands in flight)
+                                       ("d", 64, 32, 1500), ("d", 45, 45, 1100), ("s", 33, 17, 1200),
+                                       # wider than the LDS form: J/K in a global workspace
+                                       ("d", 300, 128, 5), ("d", 112, 112, 5), ("s", 256, 128, 5)])
+def test_backward_kernel_vs_closed_form(dt, m, n, nb):
     """vjh_dsvd_bwd / vjh_ssvd_bwd on their own, against the adjoint formula
     evaluated in numpy (float64) on random orthonormal U, V, distinct s and
     random cotangents: shapes that are not multiples of the 16 x 16 tiles,
-    long columns, square cases and widths past the LDS form (n > 96 FP64).
-    Relative error 1e-12 (FP64), 1e-4 (FP32) of the result's max-norm."""
+    long columns, square cases, small matrices in the staged form (m n <=
+    2048, n n <= 2048) over more matrices than workgroups, and widths past
+    the LDS form (n > 96 FP64).  Relative error 1e-12 (FP64), 1e-4 (FP32) of
+    the result's max-norm."""
     import torch
     from vecjac_amd import _lib
     from vecjac_amd import autograd  # noqa: F401  (declares the prototypes)
-    nb = 5
     g = W.rng(1000 * m + n)
     Us, Ss, Vs, dUs, dSs, dVs, ref = [], [], [], [], [], [], []
     for _ in range(nb):

@@ -272,6 +272,277 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void k_svd_bwd(const int nb, const 
   }
 }
 
+// Staged form for small matrices (every operand of one matrix in LDS): the
+// same four products and N as k_svd_bwd, with U, dU, V, dV, s and ds copied
+// into LDS once per matrix (the columns of U, dU, V and dV padded to an odd
+// leading dimension, so that the 16 columns an MFMA operand reads fall in
+// different banks) and the next matrix's copy loaded into registers while
+// this one is computed, so that the products wait on LDS, never on device
+// memory.  The two tiles a wave owns in a phase run as two independent
+// chains of MFMAs, interleaved.
+constexpr int STG_U = 8;  // staged elements per lane of U and of dU (m n <= 2048)
+constexpr int STG_V = 8;  // of V and of dV (n n <= 2048)
+constexpr int STG_LANES = 64 * BWD_WAVES;
+constexpr int STG_CH = 4; // steps of 4 rows whose LDS operands are read together
+
+struct StgLayout {
+  int ldU, ldV;             // padded leading dimensions of the staged U, dU and V, dV
+  size_t oU, odU, oV, odV;  // offsets (elements) past the k_svd_bwd part of LDS
+  size_t oS, odS, total;
+};
+
+__host__ __device__ inline StgLayout stg_layout(int m, int n)
+{
+  StgLayout L;
+  L.ldU = m | 1;
+  L.ldV = n | 1;
+  L.oU = 0;
+  L.odU = L.oU + (size_t)n * L.ldU;
+  L.oV = L.odU + (size_t)n * L.ldU;
+  L.odV = L.oV + (size_t)n * L.ldV;
+  L.oS = L.odV + (size_t)n * L.ldV;
+  L.odS = L.oS + (size_t)n;
+  L.total = L.odS + (size_t)n;
+  return L;
+}
+
+template <typename T> static bool stg_fits(int m, int n)
+{
+  return (n <= 64) && ((long long)m * n <= (long long)STG_LANES * STG_U) && (n * n <= STG_LANES * STG_V) &&
+         (bwd_lds_bytes<T>((n + 15) & ~15, 1, false) + stg_layout(m, n).total * sizeof(T) <= 160 * 1024);
+}
+
+// the interleaved pair of J / K tiles (X^T Y of columns of length len, X
+// and Y staged with leading dimension ldX) of tasks task and task2 (< 0:
+// none)
+template <typename T>
+__device__ __forceinline__ void stg_tile2(const T *X, const T *Y, const T *X2, const T *Y2, const int ldX,
+                                          const int ldX2, const int len, const int len2, const int n, const int ti,
+                                          const int tj, const int ti2, const int tj2, const bool two, const int li,
+                                          const int lk, typename Mfma<T>::V4 &acc, typename Mfma<T>::V4 &acc2)
+{
+  const int ca = ti * 16 + li, cb = tj * 16 + li, ca2 = ti2 * 16 + li, cb2 = tj2 * 16 + li;
+  const int lmax = (two && len2 > len) ? len2 : len;
+  acc = {0, 0, 0, 0};
+  acc2 = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < lmax; k0 += 4 * STG_CH) {
+    T a[STG_CH], bv[STG_CH], a2[STG_CH], b2[STG_CH];
+#pragma unroll
+    for (int c = 0; c < STG_CH; ++c) {
+      const int k = k0 + 4 * c + lk;
+      a[c] = (ca < n && k < len) ? X[ca * ldX + k] : (T)0;
+      bv[c] = (cb < n && k < len) ? Y[cb * ldX + k] : (T)0;
+      a2[c] = (two && ca2 < n && k < len2) ? X2[ca2 * ldX2 + k] : (T)0;
+      b2[c] = (two && cb2 < n && k < len2) ? Y2[cb2 * ldX2 + k] : (T)0;
+    }
+#pragma unroll
+    for (int c = 0; c < STG_CH; ++c) {
+      if (k0 + 4 * c < len)
+        acc = Mfma<T>::op(a[c], bv[c], acc);
+      if (two && (k0 + 4 * c < len2))
+        acc2 = Mfma<T>::op(a2[c], b2[c], acc2);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(STG_LANES, 2) void k_svd_bwd_stg(const int nb, const int m, const int n, const int n16,
+                                                           const T *__restrict__ U, const T *__restrict__ s,
+                                                           const T *__restrict__ V, const T *__restrict__ dU,
+                                                           const T *__restrict__ ds, const T *__restrict__ dV,
+                                                           T *__restrict__ dA)
+{
+  using V4 = typename Mfma<T>::V4;
+  extern __shared__ double smem_[];
+  const int ld = n16 + 1;
+  T *const sJ = (T *)smem_;           // J, then N
+  T *const sK = sJ + n16 * ld;        // K, then P
+  T *const sR = sK + n16 * ld;        // 1/s_k (0 past n)
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  T *const sT = sR + n16 + w * 16 * TILE_LD;
+  T *const stg = sR + n16 + BWD_WAVES * 16 * TILE_LD;
+  const StgLayout L = stg_layout(m, n);
+  T *const sU = stg + L.oU, *const sdU = stg + L.odU, *const sV = stg + L.oV, *const sdV = stg + L.odV;
+  T *const sS = stg + L.oS, *const sdS = stg + L.odS;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nt = n16 >> 4, mt = (m + 15) >> 4;
+  const int mn = m * n, nn = n * n;
+  // element t + STG_LANES j of a column-major array of R rows sits in row
+  // (t + STG_LANES j) % R of column (t + STG_LANES j) / R: the staged copy
+  // walks it by steps of STG_LANES
+  const int iU0 = t % m, kU0 = t / m, dIU = STG_LANES % m, dKU = STG_LANES / m;
+  const int iV0 = t % n, kV0 = t / n, dIV = STG_LANES % n, dKV = STG_LANES / n;
+  T pU[STG_U], pdU[STG_U], pV[STG_V], pdV[STG_V], ps = (T)0, pds = (T)0;
+  auto fetch = [&](const int b) {
+    const T *const Ub = U + (long long)b * mn, *const dUb = dU + (long long)b * mn;
+    const T *const Vb = V + (long long)b * nn, *const dVb = dV + (long long)b * nn;
+#pragma unroll
+    for (int j = 0; j < STG_U; ++j) {
+      const int e = t + STG_LANES * j;
+      pU[j] = (e < mn) ? Ub[e] : (T)0;
+      pdU[j] = (e < mn) ? dUb[e] : (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < STG_V; ++j) {
+      const int e = t + STG_LANES * j;
+      pV[j] = (e < nn) ? Vb[e] : (T)0;
+      pdV[j] = (e < nn) ? dVb[e] : (T)0;
+    }
+    ps = (t < n) ? s[(long long)b * n + t] : (T)0;
+    pds = (t < n) ? ds[(long long)b * n + t] : (T)0;
+  };
+  auto stage = [&]() {
+    int i = iU0, k = kU0;
+#pragma unroll
+    for (int j = 0; j < STG_U; ++j) {
+      if (t + STG_LANES * j < mn) {
+        sU[k * L.ldU + i] = pU[j];
+        sdU[k * L.ldU + i] = pdU[j];
+      }
+      i += dIU;
+      k += dKU;
+      if (i >= m) {
+        i -= m;
+        ++k;
+      }
+    }
+    i = iV0;
+    k = kV0;
+#pragma unroll
+    for (int j = 0; j < STG_V; ++j) {
+      if (t + STG_LANES * j < nn) {
+        sV[k * L.ldV + i] = pV[j];
+        sdV[k * L.ldV + i] = pdV[j];
+      }
+      i += dIV;
+      k += dKV;
+      if (i >= n) {
+        i -= n;
+        ++k;
+      }
+    }
+    if (t < n) {
+      sS[t] = ps;
+      sdS[t] = pds;
+    }
+  };
+  int b = blockIdx.x;
+  if (b >= nb)
+    return;
+  fetch(b);
+  stage();
+  __syncthreads();
+  for (; b < nb; b += gridDim.x) {
+    const int bn = b + gridDim.x;
+    if (bn < nb)
+      fetch(bn); // the next matrix in flight while this one is computed
+    for (int q = t; q < n16; q += STG_LANES)
+      sR[q] = (q < n) ? (T)1 / sS[q] : (T)0;
+    // J = U^T dU (inner m) and K = V^T dV (inner n), two tiles at a time
+    const int ntask = 2 * nt * nt;
+    for (int task = w; task < ntask; task += 2 * BWD_WAVES) {
+      const int task2 = task + BWD_WAVES;
+      const bool two = task2 < ntask;
+      const bool isK = task >= nt * nt, isK2 = task2 >= nt * nt;
+      const int tt = isK ? task - nt * nt : task, tt2 = isK2 ? task2 - nt * nt : task2;
+      const int ti = tt % nt, tj = tt / nt, ti2 = tt2 % nt, tj2 = tt2 / nt;
+      V4 acc, acc2;
+      stg_tile2<T>(isK ? sV : sU, isK ? sdV : sdU, isK2 ? sV : sU, isK2 ? sdV : sdU, isK ? L.ldV : L.ldU,
+                   isK2 ? L.ldV : L.ldU, isK ? n : m, isK2 ? n : m, n, ti, tj, ti2, tj2, two, li, lk, acc, acc2);
+      {
+        T *const dst = isK ? sK : sJ;
+        const int cb = tj * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          dst[cb * ld + ti * 16 + Mfma<T>::row(lk, r)] = acc[r];
+      }
+      if (two) {
+        T *const dst = isK2 ? sK : sJ;
+        const int cb = tj2 * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          dst[cb * ld + ti2 * 16 + Mfma<T>::row(lk, r)] = acc2[r];
+      }
+    }
+    __syncthreads();
+    // N in place of J, as in k_svd_bwd
+    for (int q = t; q < n16 * n16; q += STG_LANES) {
+      const int i = q % n16, j = q / n16;
+      if (i > j)
+        continue;
+      if (j >= n) {
+        sJ[j * ld + i] = (T)0;
+        sJ[i * ld + j] = (T)0;
+        continue;
+      }
+      if (i == j) {
+        sJ[i * ld + i] = sdS[i] - sJ[i * ld + i] * sR[i];
+        continue;
+      }
+      const T si = sS[i], sj = sS[j];
+      const T F = (T)1 / ((sj - si) * (sj + si));
+      const T Jij = sJ[j * ld + i], Jji = sJ[i * ld + j];
+      const T aJ = Jij - Jji;
+      const T aK = sK[j * ld + i] - sK[i * ld + j];
+      sJ[j * ld + i] = F * (aJ * sj + si * aK) - Jij * sR[j];
+      sJ[i * ld + j] = F * (aJ * si + sj * aK) - Jji * sR[i];
+    }
+    __syncthreads();
+    // P = N V^T in place of K
+    for (int tt = w; tt < nt * nt; tt += BWD_WAVES) {
+      const int ti = tt % nt, tj = tt / nt;
+      const int ia = ti * 16 + li, cb = tj * 16 + li;
+      V4 acc = {0, 0, 0, 0};
+      for (int k0 = 0; k0 < n16; k0 += 16) {
+        T a[4], bv[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int k = k0 + 4 * c + lk;
+          a[c] = sJ[k * ld + ia];
+          bv[c] = (k < n && cb < n) ? sV[k * L.ldV + cb] : (T)0;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          acc = Mfma<T>::op(a[c], bv[c], acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        sK[cb * ld + ti * 16 + Mfma<T>::row(lk, r)] = acc[r];
+    }
+    __syncthreads();
+    // dA = U P + (dU S^-1) V^T
+    for (int tt = w; tt < mt * nt; tt += BWD_WAVES) {
+      const int ri = tt % mt, cj = tt / mt;
+      const int ra = ri * 16 + li, cb = cj * 16 + li;
+      const bool rin = ra < m;
+      V4 acc = {0, 0, 0, 0};
+      for (int k0 = 0; k0 < n16; k0 += 16) {
+        T a1[4], b1[4], a2[4], b2[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int k = k0 + 4 * c + lk;
+          const bool kin = k < n;
+          a1[c] = (rin && kin) ? sU[k * L.ldU + ra] : (T)0;
+          a2[c] = (rin && kin) ? sdU[k * L.ldU + ra] * sR[k] : (T)0;
+          b2[c] = (kin && cb < n) ? sV[k * L.ldV + cb] : (T)0;
+          b1[c] = sK[cb * ld + k];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          acc = Mfma<T>::op(a1[c], b1[c], acc);
+          acc = Mfma<T>::op(a2[c], b2[c], acc);
+        }
+      }
+      store_tile<T>(dA + (long long)b * m * n, m, n, ri * 16, cj * 16, acc, sT, lane);
+    }
+    __syncthreads();
+    if (bn < nb) {
+      stage();
+      __syncthreads();
+    }
+  }
+}
+
 // Complex adjoint (split real and imaginary planes), for a loss that does
 // not depend on the phase of the singular vector pairs (u_k, v_k) ->
 // (u_k e^{i phi}, v_k e^{i phi}).  With J = U^H dU, K = V^H dV:
@@ -456,6 +727,20 @@ int svd_bwd(int nb, int m, int n, const T *U, const T *s, const T *V, const T *d
   if (n < 1 || m < n)
     return set_error(-2, "svd backward: need 1 <= n <= m");
   const int n16 = (n + 15) & ~15;
+  if (stg_fits<T>(m, n)) {
+    // small matrices: every operand staged in LDS
+    int dev = 0, cus = 0;
+    VJ_HIP(hipGetDevice(&dev));
+    VJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t lds = bwd_lds_bytes<T>(n16, 1, false) + stg_layout(m, n).total * sizeof(T);
+    const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+    const int grid = nb < per_cu * cus ? nb : per_cu * cus;
+    if (lds > 64 * 1024)
+      VJ_HIP(hipFuncSetAttribute((const void *)k_svd_bwd_stg<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_svd_bwd_stg<T>), dim3(grid), dim3(STG_LANES), lds, st, nb, m, n, n16, U, s, V, dU, ds, dV,
+                       dA);
+    return check_launch();
+  }
   BwdLaunch L{};
   if (int rc = bwd_shape<T>(nb, n16, 1, L))
     return rc;
