@@ -716,7 +716,8 @@ def bench_grad(args):
                      "note": "algorithmic bytes are the per-step column traffic; in the LDS-resident class it moves "
                              "between LDS and registers and HBM sees G in and U, V out once (about 0.17 GB per run, "
                              "profiles/k_bsvd_fp64_4096x64x32.md): the launch is bound by VALU instruction issue",
-                     "backward": {"kernel": "k_svd_bwd", "kernel_avg_ms": b_ms, "alg_bytes_per_launch": bwd_bytes,
+                     "backward": {"kernel": "k_svd_bwd_stg" if (m * n <= 2048 and n * n <= 2048) else "k_svd_bwd",
+                                  "kernel_avg_ms": b_ms, "alg_bytes_per_launch": bwd_bytes,
                                   "achieved": bwd_bytes / (b_ms / 1e3) / 1e9,
                                   "frac": bwd_bytes / (b_ms / 1e3) / 1e9 / HBM_PEAK_GBS}},
         "grad_check": {"vs": "torch.linalg.svd FP64 autograd (CPU), same gauge", "matrices": ns,
