@@ -518,30 +518,33 @@ __device__ __forceinline__ PairDec<T> pair_decide(const PairScal<T> &P, const T 
     mae = mae + getexp(maf);
     maf = getmant(maf);
     const bool cgs = eflt(mae, Mae, maf, Maf);
+    // Both columns are nonzero here (a zero column has a21 = 0), so e1, e2
+    // are finite and f1, f2 lie in [1, 2): the quotients lie in (1/2, 2), and
+    // getexp / getmant / scalef reduce to their finite, nonzero cases.
     T f12 = f1 / f2, e12 = e1 - e2, f21 = f2 / f1, e21 = e2 - e1;
-    e12 = e12 + getexp(f12);
-    f12 = getmant(f12);
-    e21 = e21 + getexp(f21);
-    f21 = getmant(f21);
+    e12 = e12 + (T)(frexp_exp(f12) - 1);
+    f12 = frexp_mant(f12) * (T)2;
+    e21 = e21 + (T)(frexp_exp(f21) - 1);
+    f21 = frexp_mant(f21) * (T)2;
     const bool c12 = efle(e12, e21, f12, f21);
     const T Ee = c12 ? e21 : e12;
     const T dd = xmin((T)Lim<T>::FIN_EXP - Ee, (T)0);
     e12 = e12 + dd;
     e21 = e21 + dd;
-    const T w0 = scalef(f12, e12), w1 = scalef(f21, e21);
+    const T w0 = scalef_fin(f12, e12), w1 = scalef_fin(f21, e21);
     T l1, l2;
     bool Pm;
     if constexpr (CPLX) {
       if (sec)
-        cjac2<T, JAC2_SEC>(w0, w1, scalef(P.a21r, dd), scalef(P.a21i, dd), d.c, d.at, d.sat, l1, l2, Pm);
+        cjac2<T, JAC2_SEC, true>(w0, w1, scalef_fin(P.a21r, dd), scalef_fin(P.a21i, dd), d.c, d.at, d.sat, l1, l2, Pm);
       else
-        cjac2<T, JAC2_TAN>(w0, w1, scalef(P.a21r, dd), scalef(P.a21i, dd), d.c, d.at, d.sat, l1, l2, Pm);
+        cjac2<T, JAC2_TAN, true>(w0, w1, scalef_fin(P.a21r, dd), scalef_fin(P.a21i, dd), d.c, d.at, d.sat, l1, l2, Pm);
     }
     else {
       if (sec)
-        rjac2<T, JAC2_SEC>(w0, w1, scalef(P.a21r, dd), d.c, d.at, l1, l2, Pm);
+        rjac2<T, JAC2_SEC, true>(w0, w1, scalef_fin(P.a21r, dd), d.c, d.at, l1, l2, Pm);
       else
-        rjac2<T, JAC2_TAN>(w0, w1, scalef(P.a21r, dd), d.c, d.at, l1, l2, Pm);
+        rjac2<T, JAC2_TAN, true>(w0, w1, scalef_fin(P.a21r, dd), d.c, d.at, l1, l2, Pm);
     }
     d.code = (!ngs && cgs) ? 3 : ((ngs && cgs) ? -3 : (Pm ? -2 : 2));
   }

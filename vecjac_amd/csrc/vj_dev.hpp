@@ -163,11 +163,13 @@ template <typename T> __device__ __forceinline__ T jac2_unscale(T x, int e)
 // Same operation sequence as D8JAC2_LOOP / D8JACL_LOOP / D8JACI_LOOP
 // (dbjac2.c:17-44, 51-78, 148-174): exact 2^es prescaling, tan(2phi) clipped
 // to sqrt(huge), tan(phi), cos, and eigenvalues unscaled by 2^-es.
-template <typename T, int FORM>
+// FIN: the caller guarantees finite inputs (the SVD's pair decisions), so
+// the special-value path is not compiled.
+template <typename T, int FORM, bool FIN = false>
 __device__ __forceinline__ void rjac2(T a11, T a22, T a21, T &c, T &at, T &l1, T &l2, bool &P)
 {
   // finite inputs (every lane but the special-value ones): integer exponents
-  const bool fin = isfinite_(a11) && isfinite_(a22) && isfinite_(a21);
+  const bool fin = FIN || (isfinite_(a11) && isfinite_(a22) && isfinite_(a21));
   int ei = 0;
   T es = (T)0, ar, a1, a2;
   if (fin) {
@@ -188,7 +190,7 @@ __device__ __forceinline__ void rjac2(T a11, T a22, T a21, T &c, T &at, T &l1, T
   }
   const T aa = vabs(ar);
   const T as = vsgn(ar);
-  const T ab = scalef(aa, (T)1);
+  const T ab = FIN ? ldexp_(aa, 1) : scalef(aa, (T)1);
   const T ad = a1 - a2;
   const T t2 = vor(xmin(xmax(ab / vabs(ad), (T)0), (T)Lim<T>::SQRT_HUGE), vsgn(ad));
   const T t1 = t2 / ((T)1 + sqrt_(fma_(t2, t2, (T)1)));
@@ -223,11 +225,11 @@ __device__ __forceinline__ void rjac2(T a11, T a22, T a21, T &c, T &at, T &l1, T
 // (Z8JAC2_LOOP / Z8JACL_LOOP / Z8JACI_LOOP, zbjac2.c:18-58, 65-105, 179-218):
 // a21 = |a21| e^{i alpha} via a safe hypot, then the real path; outputs
 // cat = cos(alpha) t, sat = sin(alpha) t (t = tan or sin per FORM).
-template <typename T, int FORM>
+template <typename T, int FORM, bool FIN = false>
 __device__ __forceinline__ void cjac2(T a11, T a22, T a21r, T a21i, T &c, T &cat, T &sat, T &l1, T &l2, bool &P)
 {
   // finite inputs (every lane but the special-value ones): integer exponents
-  const bool fin = isfinite_(a11) && isfinite_(a22) && isfinite_(a21r) && isfinite_(a21i);
+  const bool fin = FIN || (isfinite_(a11) && isfinite_(a22) && isfinite_(a21r) && isfinite_(a21i));
   int ei = 0;
   T es = (T)0, ar, ai, a1, a2;
   if (fin) {
@@ -258,7 +260,7 @@ __device__ __forceinline__ void cjac2(T a11, T a22, T a21r, T a21i, T &c, T &cat
   aa = sqrt_(fma_(aa, aa, (T)1)) * aM;
   ar_ = vor(xmin(ar_ / aa, (T)1), as);
   ai_ = ai / xmax(aa, (T)Lim<T>::TRUE_MIN);
-  const T ab = scalef(aa, (T)1);
+  const T ab = FIN ? ldexp_(aa, 1) : scalef(aa, (T)1);
   const T ad = a1 - a2;
   const T t2 = vor(xmin(xmax(ab / vabs(ad), (T)0), (T)Lim<T>::SQRT_HUGE), vsgn(ad));
   const T t1 = t2 / ((T)1 + sqrt_(fma_(t2, t2, (T)1)));
