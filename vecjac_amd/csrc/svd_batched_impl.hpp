@@ -511,11 +511,13 @@ __device__ __forceinline__ unsigned pair2(T (&x0)[E], T (&y0)[E], T (&x1)[E], T 
   pair_maxima<T, false, E>(x1, y1, zi, zi, nE, mx1, my1);
   red_max4(mx0, my0, mx1, my1);
   const int kx0 = max_exp(mx0), ky0 = max_exp(my0), kx1 = max_exp(mx1), ky1 = max_exp(my1);
+  // six sums (the imaginary parts of the dots are zero): two reduced while
+  // the second pair's are formed, then four
   DD<A> sx0{0, 0}, sy0{0, 0}, dr0{0, 0}, di0{0, 0}, sx1{0, 0}, sy1{0, 0}, dr1{0, 0}, di1{0, 0};
   pair_sums<T, false, E>(x0, y0, zi, zi, nE, kx0, ky0, sx0, sy0, dr0, di0);
-  red_dd4<A, B>(sx0, sy0, dr0, di0, nullptr);
+  red_dd2<A>(sx0, sy0);
   pair_sums<T, false, E>(x1, y1, zi, zi, nE, kx1, ky1, sx1, sy1, dr1, di1);
-  red_dd4<A, B>(sx1, sy1, dr1, di1, nullptr);
+  red_dd4<A, B>(dr0, sx1, sy1, dr1, nullptr);
   const bool t1 = (lane & 1) != 0;
   const PairScal<T> Pl = pair_scal<T, false>(t1 ? mx1 : mx0, t1 ? my1 : my0, t1 ? kx1 : kx0, t1 ? ky1 : ky0,
                                              dd_sel(t1, sx1, sx0), dd_sel(t1, sy1, sy0), dd_sel(t1, dr1, dr0),

@@ -217,6 +217,22 @@ __device__ __forceinline__ void red_max2(T &a, T &b, double *lds)
   }
 }
 
+// Sum of two double-double values over a wave (64 lanes).  Each value goes
+// through the same additions, in the same order, as a value of red_dd4: lane
+// l plus lane l + 32 (swap32), the two rows of each half (swap16), then the
+// row reduction; so its totals are bit-identical to red_dd4's, at half the
+// exchanges and additions.
+template <typename A> __device__ __forceinline__ void red_dd2(DD<A> &q0, DD<A> &q1)
+{
+  const DD<A> k = dd_swapadd<true>(q0, q1);          // low half: q0 partials, high half: q1 partials
+  const DD<A> r = row_sum(dd_swapadd<false>(k, k));  // rows 0, 1: q0; rows 2, 3: q1
+  DD<A> a = r, b = r;
+  swap_<true>(a.s, b.s);                             // a: rows (0, 1, 0, 1), b: rows (2, 3, 2, 3)
+  swap_<true>(a.c, b.c);
+  q0 = a;
+  q1 = b;
+}
+
 // Sum of four double-double values over the group.  Wave level: swap32
 // scatters (q0, q2) and (q1, q3) over the halves, swap16 scatters the two
 // results over the rows (row r then holds a partial sum of q_r), the rows
