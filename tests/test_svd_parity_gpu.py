@@ -251,14 +251,16 @@ def test_square_full_width(n):
 
 # --------------------------------- V preset ---------------------------------
 
-@pytest.mark.parametrize("kind", ["d", "z"])
-def test_vpreset_resume(kind):
+@pytest.mark.parametrize("kind,m,n", [("d", 256, 64), ("z", 256, 64),
+                                    # columns streamed in chunks
+                                    ("d", 9000, 8), ("z", 5000, 8)])
+def test_vpreset_resume(kind, m, n):
     """A run stopped after two sweeps returns its working G (A V 2^sT) and
     V; the driver resumed from those with a preset V matches the oracle's
     iwork[0] = 1 path on the same G and V (sweeps within one, sigma within
     8 n eps, both modes) and an uninterrupted run (sigma 2^-sT within
-    8 n eps), with A = U S V^H."""
-    m, n = 256, 64
+    8 n eps), with A = U S V^H; also for columns long enough to be
+    streamed."""
     g_ = W.rng(43)
     A = g_.standard_normal((m, n))
     if kind == "z":

@@ -544,14 +544,23 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     }
     const PairScal<T> P = pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
     D = pair_decide<T, CPLX>(P, S.tol, S.gst, S.sec);
-    T tG = (T)0;
-    for (int c = 0; c < nc; ++c) {
+    // The update visits the chunks last to first: the ones the sums read
+    // last are the likeliest to be still in the caches (the update of an
+    // element does not depend on the others, so the order changes no bits;
+    // the running maximum of the updated elements is order-independent for
+    // the finite values a run produces).
+    T tG = (T)0, tV = (T)0;
+    for (int cc = 0; cc < nc; ++cc) {
+      const int c = nc - 1 - cc;
       load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
-      // the V pair is updated with the first chunk
-      o = pair_apply<T, CPLX, B, E, false>(c ? 0 : n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, S.vtrack, eSp, fSp, eSq,
+      // the V pair is updated with the first chunk visited (whose call
+      // reports the V maxima of a preset V)
+      o = pair_apply<T, CPLX, B, E, false>(cc ? 0 : n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, S.vtrack, eSp, fSp, eSq,
                                            fSq, lds, lane, E);
       if (o.err)
         break;
+      if (cc == 0)
+        tV = o.tV;
       tG = xmax(tG, o.tG);
       if ((o.code != 1) || S.sG)
         store_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, lane, xr, yr, xi, yi);
@@ -559,6 +568,7 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     if (!o.err) {
       tG = red_max<T, B>(tG, lds);
       o.tG = tG;
+      o.tV = tV;
       if (((o.code == 2) || (o.code == -2) || (o.code == 3) || (o.code == -3)) && !isfinite_(tG))
         o.err = CPLX ? -630 : -580;
     }
