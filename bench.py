@@ -448,8 +448,9 @@ def _bsvd_cpu(m_real, n_real, m, offG, npad, G64, kind):
 
 def bench_plain(args, kind: str):
     """Plain SVD of one large matrix through the device driver (one launch
-    per step): cfg1 (dvjsvd, 1024 x 1024 N(0,1)) or cfg3 (zvjsvd, 8192 x 2048,
-    A = Q1 diag(sigma) Q2^H, sigma geometric from 1 to 1e-12)."""
+    per step): dvjsvd on a 1024 x 1024 N(0,1) matrix, or zvjsvd on an
+    8192 x 2048 complex A = Q1 diag(sigma) Q2^H with sigma geometric from 1
+    to 1e-12."""
     import torch
     from vecjac_amd import svd as S
     from vecjac_amd import workloads as W
@@ -783,8 +784,8 @@ def main() -> int:
     ap.add_argument("--order", type=int, default=0, help="pjs id of the batched ordering (4 or 6; 0: the plan's)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the batched SVD shards by matrix with no communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this benchmark measures one GPU only (asked for --gpus {args.gpus}); "
+              "the batched SVD would shard by matrix with no communication", file=sys.stderr)
         return 2
     import torch
     if not torch.cuda.is_available():

@@ -4,7 +4,7 @@
  * from its sources with its JTRACE output (_ref/libvjref_jtrace.so), on a
  * matrix file (m x n doubles, column-major) and a step-table file (stp x n
  * unsigned, stp = size / 4n), writing the reference's trace to TraceFile.
- * Prints the return value.  Used to check this build's trace files.
+ * Prints the return value.  Used to check the GPU library's trace files.
  */
 #include <stdint.h>
 #include <stdio.h>

@@ -1,7 +1,7 @@
 /* Oracle (test infrastructure only): CPU restatement of the reference's
  * one-sided Jacobi SVD and batched 2x2 Hermitian EVD.  Only tests/, the
  * smoke check and the benchmark's CPU-baseline leg may load liboracle_vj.so,
- * and only as the checker, never as the measured or shipped path.
+ * and only to check results, never as the measured or shipped path.
  *
  * Names mirror the reference with a vjo_ prefix and no trailing underscore;
  * arguments are passed the way the reference passes them (by pointer for the

@@ -2,7 +2,7 @@
 CPU restatement of the reference, and of oracle/_ref/libvjref.so, the
 reference's own hot-path sources compiled in place (when they were present
 at build time).  Only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg may import this module, and only as the checker.
+cpu_baseline leg may import this module, and only to check results.
 """
 from __future__ import annotations
 

@@ -170,23 +170,23 @@ def test_golden_reference_outputs(name):
 
 # --------------------------- graded constructions ---------------------------
 
-def _cfg3_input(m, n, seed=7):
+def _graded_geometric_input(m, n, seed=7):
     sig = 10.0 ** (-12.0 * np.arange(n) / (n - 1))
     return W.graded_svd_matrix(m, n, sig, seed, True)
 
 
 @pytest.mark.timeout(300)
 @pytest.mark.parametrize("m,n", [(512, 128), (1024, 256)])
-def test_graded_cfg3_construction(m, n):
+def test_graded_geometric_sigma(m, n):
     """A = Q1 diag(sigma) Q2^H, complex, sigma = 10^(-12 i/(n-1)), the ME
     table; bars as in the module docstring."""
-    A = _cfg3_input(m, n)
+    A = _graded_geometric_input(m, n)
     Ar, Ai = _split(A)
     js, _ = S.pjs(2, n)
     g = _gpu("z", Ar, Ai, js)
     assert g["rc"] > 0
     sg = _sig(g)
-    print(f"graded cfg3 construction {m}x{n}")
+    print(f"graded construction (sigma geometric 1 .. 1e-12) {m}x{n}")
     for mode in (1, 0):
         o = _oracle("z", Ar, Ai, js, mode)
         so = _sig(o)
