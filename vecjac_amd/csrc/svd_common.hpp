@@ -55,6 +55,7 @@ template <typename T> struct PlainState {
   int sT, sV;                    // accumulated power-of-two shifts
   int sG_init, sV_init;          // initial scalings
   unsigned sweeps;               // completed sweeps with transformations
+  unsigned vsweeps;              // sweeps whose steps ran (deferred V updates)
   unsigned long long swt;        // transformations in the current sweep
   unsigned long long hist[64];   // transformations per sweep (trace)
   // trace of the scalings (the reference's JTRACE lines, dvjsvd.c:70-635)

@@ -52,6 +52,7 @@ __global__ void k_setup(const int m, PlainState<T> *st, const int vpreset)
   st->ctl[0].sG = st->ctl[0].sV = 0;
   st->sG_init = st->sV_init = 0;
   st->sweeps = 0;
+  st->vsweeps = 0;
   st->swt = 0;
   st->zero = 0;
   st->status = 0;
@@ -123,6 +124,188 @@ __global__ __launch_bounds__(256) void k_init_cols(const int m, const int n, T *
   if (threadIdx.x == 0) {
     eS[j] = -inf_<T>();
     fS[j] = (T)1;
+  }
+}
+
+// -------------------------------------------------- deferred V updates
+// With V := I (no preset V, so no tracking or rescaling of V) the steps
+// leave V alone and record each pair's decision; after every sweep k_vflush
+// applies the sweep's steps to V in order, R rows at a time held in LDS.  An
+// element of V only meets the element of the same row in the other column of
+// its pair, so each goes through the same operations in the same order as
+// in the steps: the same bits, for one pass over V per sweep instead of a
+// read and a write of both V columns of every transformed pair at every step.
+template <typename T> struct VRec {
+  int *code; // action code per step and pair (nullptr: V updated in the steps)
+  T *c, *at, *sat;
+};
+
+// Workgroup barrier that orders the LDS accesses only (global loads in
+// flight stay in flight).
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One element pair of V under a recorded decision, as pair_apply updates
+// the V columns (codes 2 and -2: rotation, with a zero tangent a swap when
+// the cosine is 1 and the pair is permuted; -1 and -3: swap; 1 and 3: none).
+template <typename T, bool CPLX>
+__device__ __forceinline__ void vflush_elem(const int code, const T c, const T at, const T sat, const int sec, T &xr,
+                                            T &yr, T &xi, T &yi)
+{
+  bool swap = (code == -1) || (code == -3);
+  if ((code == 2) || (code == -2)) {
+    const bool perm = (code < 0);
+    if ((at == (T)0) && (!CPLX || (sat == (T)0)))
+      swap = (c == (T)1) && perm;
+    else {
+      const bool c1 = (c == (T)1);
+      const T x_r = xr, y_r = yr;
+      if (!CPLX || (sat == (T)0)) {
+        T a = perm ? fma_(-x_r, at, y_r) : fma_(y_r, at, x_r);
+        T b = perm ? fma_(y_r, at, x_r) : fma_(-x_r, at, y_r);
+        if (!c1) { a = cscale(a, c, sec); b = cscale(b, c, sec); }
+        xr = a;
+        yr = b;
+        if constexpr (CPLX) {
+          const T x_i = xi, y_i = yi;
+          T a2 = perm ? fma_(-x_i, at, y_i) : fma_(y_i, at, x_i);
+          T b2 = perm ? fma_(y_i, at, x_i) : fma_(-x_i, at, y_i);
+          if (!c1) { a2 = cscale(a2, c, sec); b2 = cscale(b2, c, sec); }
+          xi = a2;
+          yi = b2;
+        }
+      }
+      else if constexpr (CPLX) {
+        const T cna = -at;
+        const T x_i = xi, y_i = yi;
+        T ar = fma_(y_r, at, fma_(-y_i, sat, x_r));
+        T ai = fma_(y_r, sat, fma_(y_i, at, x_i));
+        T br = fma_(x_r, cna, fma_(-x_i, sat, y_r));
+        T bi = fma_(x_r, sat, fma_(x_i, cna, y_i));
+        if (!c1) { ar = cscale(ar, c, sec); ai = cscale(ai, c, sec); br = cscale(br, c, sec); bi = cscale(bi, c, sec); }
+        if (perm) { xr = br; xi = bi; yr = ar; yi = ai; }
+        else { xr = ar; xi = ai; yr = br; yi = bi; }
+      }
+    }
+  }
+  if (swap) {
+    T t = xr; xr = yr; yr = t;
+    if constexpr (CPLX) { t = xi; xi = yi; yi = t; }
+  }
+}
+
+// A pair's recorded decision for one step, with its column indices.
+template <typename T> struct VFDec {
+  bool ok; // the pair and the step exist
+  int code;
+  unsigned p, q;
+  T c, at, sat;
+};
+
+constexpr int VF_LDS_MAX = 64 * 1024; // LDS per workgroup (two per CU)
+constexpr int VF_DEPTH = 2;           // steps whose decisions are loaded ahead
+
+// Rows [R blockIdx.x, R blockIdx.x + R) of V through the steps of sweep sw
+// (when its steps ran: st->vsweeps == sw + 1; after a failing step only the
+// steps up to it).  LDS holds the rows column by column (element (r, j) at
+// j R + r); the lanes take the pairs of a step (PM each), so a step's
+// element pairs are independent and one LDS barrier separates the steps.
+// The decisions of the next step are loaded while a step is applied.
+template <typename T, bool CPLX, int PM>
+__global__ __launch_bounds__(1024) void k_vflush(const int nv, const int n, const int R, T *__restrict__ Vr,
+                                                 T *__restrict__ Vi, const long long ldV,
+                                                 const unsigned *__restrict__ js, const unsigned stp, const VRec<T> rec,
+                                                 const PlainState<T> *__restrict__ st, const unsigned sw, const int sec)
+{
+  extern __shared__ double vf_lds[];
+  if (st->vsweeps != sw + 1u)
+    return;
+  const unsigned nst = (st->status && (st->err_sw == sw)) ? st->err_st + 1u : stp;
+  if (nst == 0u)
+    return;
+  T *const br = (T *)vf_lds;
+  T *const bi = br + (size_t)R * n;
+  const int i0 = blockIdx.x * R;
+  const int rows = (nv - i0 < R) ? (nv - i0) : R;
+  const int lane = threadIdx.x, nt = blockDim.x;
+  for (int t = lane; t < R * n; t += nt) {
+    const int r = t % R, j = t / R;
+    const bool in = (r < rows);
+    br[t] = in ? Vr[(long long)j * ldV + i0 + r] : (T)0;
+    if constexpr (CPLX)
+      bi[t] = in ? Vi[(long long)j * ldV + i0 + r] : (T)0;
+  }
+  const int np = n >> 1;
+  // VF_DEPTH sets of decisions in registers: a step is applied while the
+  // loads of the next VF_DEPTH - 1 steps are in flight
+  VFDec<T> d[VF_DEPTH][PM];
+  auto fetch = [&](const unsigned s, VFDec<T>(&e)[PM]) {
+#pragma unroll
+    for (int i = 0; i < PM; ++i) {
+      const int k = lane + i * nt;
+      const bool ok = (k < np) && (s < nst);
+      const long long ix = ok ? (long long)s * np + k : 0;
+      const size_t jx = ok ? (size_t)s * n + 2 * k : 0;
+      e[i].ok = ok;
+      e[i].code = rec.code[ix];
+      e[i].c = rec.c[ix];
+      e[i].at = rec.at[ix];
+      e[i].sat = CPLX ? rec.sat[ix] : (T)0;
+      e[i].p = js[jx];
+      e[i].q = js[jx + 1];
+    }
+  };
+  auto apply = [&](const VFDec<T>(&e)[PM]) {
+#pragma unroll
+    for (int i = 0; i < PM; ++i) {
+      const int code = e[i].code;
+      if (!e[i].ok || (code == 1) || (code == 3))
+        continue;
+      T *const xp = br + (size_t)e[i].p * R, *const yp = br + (size_t)e[i].q * R;
+      T *const xpi = bi + (size_t)e[i].p * R, *const ypi = bi + (size_t)e[i].q * R;
+      for (int r = 0; r < R; ++r) {
+        T xr = xp[r], yr = yp[r], xi = (T)0, yi = (T)0;
+        if constexpr (CPLX) {
+          xi = xpi[r];
+          yi = ypi[r];
+        }
+        vflush_elem<T, CPLX>(code, e[i].c, e[i].at, e[i].sat, sec, xr, yr, xi, yi);
+        xp[r] = xr;
+        yp[r] = yr;
+        if constexpr (CPLX) {
+          xpi[r] = xi;
+          ypi[r] = yi;
+        }
+      }
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < VF_DEPTH; ++u) {
+    fetch((unsigned)u, d[u]);
+    __builtin_amdgcn_sched_barrier(0); // the loads issue in step order
+  }
+  __syncthreads();
+  // (a step past the last one applies nothing: every lane runs the same
+  // barriers, and the loads are issued in one order on every path)
+  for (unsigned s = 0; s < nst; s += VF_DEPTH) {
+#pragma unroll
+    for (int u = 0; u < VF_DEPTH; ++u) {
+      apply(d[u]);
+      lds_barrier();
+      fetch(s + VF_DEPTH + u, d[u]);
+    }
+  }
+  for (int t = lane; t < R * n; t += nt) {
+    const int r = t % R, j = t / R;
+    if (r < rows) {
+      Vr[(long long)j * ldV + i0 + r] = br[t];
+      if constexpr (CPLX)
+        Vi[(long long)j * ldV + i0 + r] = bi[t];
+    }
   }
 }
 
@@ -378,9 +561,11 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
                                              const int vtrack, const int stats, const int sec,
                                              PlainState<T> *__restrict__ st,
                                              const unsigned r, const PlainSlot *__restrict__ prev,
-                                             PlainSlot *__restrict__ slots)
+                                             PlainSlot *__restrict__ slots, const VRec<T> vrec)
 {
-  // n is the length of the V columns (the table row has gridDim.x pairs).
+  // n is the length of the V columns (the table row has gridDim.x pairs);
+  // n = 0 when the V updates are deferred (vrec.code set): the step then
+  // records its decision for k_vflush instead.
   // The control state before this step is ctl[r] with, for st_idx > 0, the
   // slots of step st_idx - 1 (prev) folded in; workgroup 0 stores the fold
   // in ctl[r ^ 1].
@@ -573,6 +758,18 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
         o.err = CPLX ? -630 : -580;
     }
   }
+  if ((lane == 0) && vrec.code) {
+    // the V update this pair owes, applied by k_vflush after the sweep (the
+    // steps update V unless the decision failed)
+    const long long ix = (long long)st_idx * gridDim.x + k;
+    vrec.code[ix] = D.err ? 1 : D.code;
+    vrec.c[ix] = D.c;
+    vrec.at[ix] = D.at;
+    if constexpr (CPLX)
+      vrec.sat[ix] = D.sat;
+    if ((st_idx == 0u) && (k == 0))
+      st->vsweeps += 1u;
+  }
   if (lane == 0) {
     // this pair's report, read by the next launch
     PlainSlot *const sl = slots + blockIdx.x;
@@ -629,17 +826,20 @@ __global__ __launch_bounds__(256) void k_epilogue(const int m, const int n, T *_
 // ------------------------------------------------------------------ driver
 template <typename T, bool CPLX, int B, int E, bool STREAM = false>
 static int launch_steps(const PlainArgs<T> &a, PlainState<T> *st, unsigned r, const PlainSlot *prev, PlainSlot *slots,
-                        const unsigned *row, unsigned s, const T tol, const T gst, hipStream_t strm)
+                        const unsigned *row, unsigned s, const T tol, const T gst, const VRec<T> &vrec,
+                        hipStream_t strm)
 {
-  hipLaunchKernelGGL((k_step<T, CPLX, B, E, STREAM>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m,
-                     (int)(a.nv ? a.nv : a.n), a.Gr, a.Gi, a.ldG, a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp,
-                     tol, gst, a.vpreset, a.stats, a.sec, st, r, prev, slots);
+  // with deferred V updates the steps see V columns of length 0
+  const int nv = vrec.code ? 0 : (int)(a.nv ? a.nv : a.n);
+  hipLaunchKernelGGL((k_step<T, CPLX, B, E, STREAM>), dim3(a.n / 2), dim3(B), 0, strm, (int)a.m, nv, a.Gr, a.Gi,
+                     a.ldG, a.Vr, a.Vi, a.ldV, a.eS, a.fS, row, s, a.stp, a.swp, tol, gst, a.vpreset, a.stats, a.sec,
+                     st, r, prev, slots, vrec);
   return 0;
 }
 
 template <typename T, bool CPLX>
 using StepFn = int (*)(const PlainArgs<T> &, PlainState<T> *, unsigned, const PlainSlot *, PlainSlot *,
-                       const unsigned *, unsigned, T, T, hipStream_t);
+                       const unsigned *, unsigned, T, T, const VRec<T> &, hipStream_t);
 
 template <typename T, bool CPLX>
 static StepFn<T, CPLX> pick_step(long long m, int &B_out)
@@ -673,6 +873,51 @@ static StepFn<T, CPLX> pick_step(long long m, int &B_out)
 }
 
 template <typename T> static T ldexp_host(T x, int e) { return std::ldexp(x, e); }
+
+// Shape of the deferred V updates: R rows of V per workgroup (the most, up
+// to 16, that keep R n elements within the LDS budget and at least two
+// workgroups per CU), nt lanes, PM pairs per lane.  False when a row of V
+// does not fit the budget or a step has more than 4096 pairs: the steps then
+// update V themselves.
+struct VFlushPlan {
+  int R, nt, pm;
+};
+
+template <typename T, bool CPLX> static bool vflush_plan(long long n, long long nv, int cus, VFlushPlan &f)
+{
+  const long long row = n * (long long)sizeof(T) * (CPLX ? 2 : 1);
+  const long long np = n / 2;
+  if ((row > VF_LDS_MAX) || (np > 4096) || (np < 1))
+    return false;
+  f.nt = (int)std::min<long long>(1024, (np + 63) / 64 * 64);
+  f.pm = (int)((np + f.nt - 1) / f.nt);
+  if (f.pm == 3)
+    f.pm = 4;
+  f.R = 1;
+  while ((f.R < 16) && (2 * f.R * row <= VF_LDS_MAX) && (nv / (2 * f.R) >= 2LL * cus))
+    f.R *= 2;
+  return true;
+}
+
+template <typename T, bool CPLX>
+static int launch_vflush(const PlainArgs<T> &a, const VFlushPlan &f, const VRec<T> &vrec, const PlainState<T> *st,
+                         unsigned sw, hipStream_t s)
+{
+  const long long nv = a.nv ? a.nv : a.n;
+  const unsigned grid = (unsigned)((nv + f.R - 1) / f.R);
+  const size_t lds = (size_t)f.R * (size_t)a.n * sizeof(T) * (CPLX ? 2 : 1);
+#define VJ_VFLUSH(PM)                                                                                              \
+  hipLaunchKernelGGL((k_vflush<T, CPLX, PM>), dim3(grid), dim3(f.nt), lds, s, (int)nv, (int)a.n, f.R, a.Vr, a.Vi, \
+                     a.ldV, a.js, a.stp, vrec, st, sw, a.sec)
+  if (f.pm == 1)
+    VJ_VFLUSH(1);
+  else if (f.pm == 2)
+    VJ_VFLUSH(2);
+  else
+    VJ_VFLUSH(4);
+#undef VJ_VFLUSH
+  return check_launch();
+}
 
 static __global__ void k_zero_words(unsigned *p, size_t words)
 {
@@ -734,6 +979,34 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s, const PlainWo
     return rc;
   const T tol = (a.tol > (T)0) ? a.tol : host_tol<T>(a.m);
   const T gst = host_gst<T>(tol);
+  // V := I and no caller's workspace: the V updates of each sweep are
+  // deferred to one pass over V after it (same bits)
+  VRec<T> vrec{nullptr, nullptr, nullptr, nullptr};
+  VFlushPlan vfp{0, 0, 0};
+  if (!a.vpreset && !work) {
+    int dev = 0, cus = 0;
+    VJ_HIP(hipGetDevice(&dev));
+    VJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (vflush_plan<T, CPLX>(a.n, nv, cus, vfp)) {
+      const size_t nrec = (size_t)a.stp * np;
+      if (int rc = sg.alloc(&vrec.code, nrec))
+        return rc;
+      if (int rc = sg.alloc(&vrec.c, nrec))
+        return rc;
+      if (int rc = sg.alloc(&vrec.at, nrec))
+        return rc;
+      if (CPLX) {
+        if (int rc = sg.alloc(&vrec.sat, nrec))
+          return rc;
+      }
+      const size_t lds = (size_t)vfp.R * (size_t)a.n * sizeof(T) * (CPLX ? 2 : 1);
+      if (lds > 65536u) {
+        const void *fn = (vfp.pm == 1) ? (const void *)k_vflush<T, CPLX, 1>
+                         : (vfp.pm == 2) ? (const void *)k_vflush<T, CPLX, 2> : (const void *)k_vflush<T, CPLX, 4>;
+        VJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      }
+    }
+  }
   // Sweeps are enqueued one ahead of the convergence check: the host only
   // reads the device flag of sweep k-1 after sweep k has been queued.  With a
   // caller's workspace the flag of each sweep is read synchronously instead.
@@ -756,7 +1029,8 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s, const PlainWo
   unsigned r = 0;
   for (unsigned sw = 0; !rc && sw < a.swp; ++sw) {
     for (unsigned k = 0; k < a.stp; ++k) {
-      step(a, st, r, slots + ((k + 1u) & 1u) * np, slots + (k & 1u) * np, a.js + (size_t)k * a.n, k, tol, gst, s);
+      step(a, st, r, slots + ((k + 1u) & 1u) * np, slots + (k & 1u) * np, a.js + (size_t)k * a.n, k, tol, gst, vrec,
+           s);
       if (k)
         r ^= 1u;
     }
@@ -764,6 +1038,8 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s, const PlainWo
                        a.stp, a.swp, a.stats);
     r ^= 1u;
     if ((rc = check_launch()))
+      break;
+    if (vrec.code && (rc = launch_vflush<T, CPLX>(a, vfp, vrec, st, sw, s)))
       break;
     ++launched;
     if (work) {
