@@ -448,9 +448,12 @@ def _bsvd_cpu(m_real, n_real, m, offG, npad, G64, kind):
 
 def bench_plain(args, kind: str):
     """Plain SVD of one large matrix through the device driver (one launch
-    per step): dvjsvd on a 1024 x 1024 N(0,1) matrix, or zvjsvd on an
-    8192 x 2048 complex A = Q1 diag(sigma) Q2^H with sigma geometric from 1
-    to 1e-12."""
+    per step, the V updates of each sweep applied in one pass after it):
+    dvjsvd on a 1024 x 1024 N(0,1) matrix, or zvjsvd on an 8192 x 2048
+    complex A = Q1 diag(sigma) Q2^H with sigma geometric from 1 to 1e-12.
+    The roofline numerator is the algorithm's bytes (every pair evaluation
+    reads its two G columns, every transformation writes them and reads and
+    writes its two V columns) over the whole run.""" 
     import torch
     from vecjac_amd import svd as S
     from vecjac_amd import workloads as W
@@ -501,7 +504,8 @@ def bench_plain(args, kind: str):
            "config": {"workload": desc, "sweeps": int(sweeps), "launched_sweeps": int(hist[1]),
                       "transformations": R},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step (all launches)",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "k_step, k_fold and k_vflush (the whole run)",
                         "kernel_avg_ms": wall / max(1, int(hist[1]) * stp) * 1e3,
                         "alg_bytes_per_launch": alg / max(1, steps_run)}}
     sg = np.sort(S.sigma(eS.cpu().numpy(), fS.cpu().numpy()))[::-1]

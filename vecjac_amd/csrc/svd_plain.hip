@@ -744,7 +744,13 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     // the running maximum of the updated elements is order-independent for
     // the finite values a run produces).
     T tG = (T)0, tV = (T)0;
-    for (int cc = 0; cc < nc; ++cc) {
+    // A pair left as it is (code 1, no pending rescaling) is not read again:
+    // its update pass would change and store nothing (its maxima are 0).
+    const bool idle = !D.err && (D.code == 1) && !S.sG;
+    if (idle)
+      o = pair_apply<T, CPLX, B, E, false>(0, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, S.vtrack, eSp, fSp, eSq, fSq,
+                                           lds, lane, 0);
+    for (int cc = 0; cc < (idle ? 0 : nc); ++cc) {
       const int c = nc - 1 - cc;
       load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
       // the V pair is updated with the first chunk visited (whose call
