@@ -208,7 +208,12 @@ vjh_int vjh_cbjac2i(vjh_int n, const float *a11, const float *a22, const float *
  * permutation).  The complex forms take one leading dimension for both
  * planes.  vpreset != 0 is the reference's iwork[0] != 0.  stats, if not
  * NULL, receives 66 words: sweeps, sweeps launched, and the number of
- * transformations in each of the first 64 sweeps.  Synchronises `stream`. */
+ * transformations in each of the first 64 sweeps.  With vpreset == 0 the
+ * V updates of each sweep are applied in one pass after it, from per-pair
+ * decision records (stp * n/2 of them, each 2 values of the precision,
+ * 3 for complex, and a 4-byte code), allocated in stream order with the
+ * call; the results are bit for bit those of updating V in every step.
+ * Synchronises `stream`. */
 vjh_int vjh_dvjsvd(vjh_nat m, vjh_nat n, double *G, vjh_nat ldG, double *V, vjh_nat ldV, double *eS, double *fS,
                    const unsigned *js, unsigned stp, unsigned swp, unsigned vpreset, void *stream, unsigned *stats);
 vjh_int vjh_zvjsvd(vjh_nat m, vjh_nat n, double *Gr, double *Gi, vjh_nat ldG, double *Vr, double *Vi, vjh_nat ldV,
