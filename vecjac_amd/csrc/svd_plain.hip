@@ -393,25 +393,80 @@ struct SlotTotals {
   unsigned long long key; // (pair index << 32 | code) of the first failing pair, ~0 if none
 };
 
-template <int B>
+// The fields of SP slots per lane that every fold reads (error code,
+// maxima, transformation flag), requested together at the start of a step:
+// they do not depend on the step's pair, and no load waits for another (the
+// error code does not gate the loads of the maxima).
+template <int SP> struct SlotPre {
+  int err[SP];
+  unsigned trans[SP];
+  double tG[SP], tV[SP];
+};
+template <> struct SlotPre<0> {};
+
+template <int B, int SP>
+__device__ __forceinline__ void prefetch_slots(const PlainSlot *__restrict__ sl, const int np, const int lane,
+                                               SlotPre<SP> &pf)
+{
+#pragma unroll
+  for (int u = 0; u < SP; ++u) {
+    const int i = lane + u * B;
+    const PlainSlot *const e = sl + ((i < np) ? i : 0);
+    pf.err[u] = e->err;
+    pf.trans[u] = e->trans;
+    pf.tG[u] = e->tG;
+    pf.tV[u] = e->tV;
+  }
+}
+
+// Marks the prefetched slots as received at this point: their loads stay
+// above it (they would otherwise be sunk to their first use, after the
+// pair's columns are requested) and their use below it.
+template <int SP> __device__ __forceinline__ void pin_slots(SlotPre<SP> &pf)
+{
+  if constexpr (SP > 0) {
+#pragma unroll
+    for (int u = 0; u < SP; ++u)
+      asm volatile("" : "+v"(pf.err[u]), "+v"(pf.trans[u]), "+v"(pf.tG[u]), "+v"(pf.tV[u]));
+  }
+}
+
+__device__ __forceinline__ void fold_slot(SlotTotals &t, const PlainSlot *__restrict__ sl, const int i, const int e,
+                                          const double tG, const double tV, const unsigned trans, const int stats)
+{
+  if (e) {
+    const unsigned long long key = ((unsigned long long)(unsigned)i << 32) | (unsigned)e;
+    t.key = (key < t.key) ? key : t.key;
+    return;
+  }
+  t.tG = fmax(t.tG, tG);
+  t.tV = fmax(t.tV, tV);
+  t.trans += trans;
+  if (stats) {
+    t.cosv = fmax(t.cosv, sl[i].cosv);
+    t.sinv = fmax(t.sinv, sl[i].sinv);
+  }
+}
+
+// Slots [0, np): the first SP B from pf, the rest loaded here.
+template <int B, int SP>
 __device__ __forceinline__ SlotTotals reduce_slots(const PlainSlot *__restrict__ sl, const int np, const int stats,
-                                                   const int lane, double *lds)
+                                                   const int lane, double *lds, const SlotPre<SP> &pf)
 {
   SlotTotals t{0.0, 0.0, 0.0, 0.0, 0ull, ~0ull};
-  for (int i = lane; i < np; i += B) {
+  if constexpr (SP > 0) {
+#pragma unroll
+    for (int u = 0; u < SP; ++u) {
+      const int i = lane + u * B;
+      if (i < np)
+        fold_slot(t, sl, i, pf.err[u], pf.tG[u], pf.tV[u], pf.trans[u], stats);
+    }
+  }
+  for (int i = lane + SP * B; i < np; i += B) {
     const int e = sl[i].err;
-    if (e) {
-      const unsigned long long key = ((unsigned long long)(unsigned)i << 32) | (unsigned)e;
-      t.key = (key < t.key) ? key : t.key;
-      continue;
-    }
-    t.tG = fmax(t.tG, sl[i].tG);
-    t.tV = fmax(t.tV, sl[i].tV);
-    t.trans += sl[i].trans;
-    if (stats) {
-      t.cosv = fmax(t.cosv, sl[i].cosv);
-      t.sinv = fmax(t.sinv, sl[i].sinv);
-    }
+    const unsigned tr = sl[i].trans;
+    const double g = sl[i].tG, v = sl[i].tV;
+    fold_slot(t, sl, i, e, g, v, tr, stats);
   }
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) {
@@ -551,7 +606,7 @@ __global__ __launch_bounds__(256) void k_fold(PlainState<T> *__restrict__ st, co
       st->ctl[r ^ 1u] = st->ctl[r];
     return;
   }
-  const SlotTotals t = reduce_slots<256>(prev, np, stats, lane, lds);
+  const SlotTotals t = reduce_slots<256, 0>(prev, np, stats, lane, lds, SlotPre<0>{});
   if (lane == 0)
     fold_commit<T>(st, r, t, fold_ctl<T, CPLX>(st->ctl[r], t, st->NRM_EXP), stp - 1u, stp, swp, stats);
 }
@@ -586,9 +641,23 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   // failing pair, by workgroup 0 of this one, and then every workgroup finds
   // that failure in the slots itself.
   const int k = blockIdx.x;
+  // the previous step's slots are requested first: they do not depend on
+  // the pair, so their latency overlaps that of the pair's indices and then
+  // of its columns
+  // (unconditionally: the first step of a sweep reads the other half of the
+  // slot buffer, allocated and unused, so that no branch separates the
+  // loads from their use)
+  constexpr int SP = (B >= 256) ? 2 : 4;
+  const int np = (int)gridDim.x;
+  asm volatile("" ::"s"(row), "s"(st), "s"(prev), "s"(np)); // the arguments used first, read in one batch
+  SlotPre<SP> pf;
+  prefetch_slots<B, SP>(prev, np, lane, pf);
   const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
   const int done = st->done;
-  asm volatile("" ::"s"(p), "s"(q), "s"(done)); // all three loads issued before the test below
+  // all three loads issued before the test below, and the slots received
+  // with them (they were requested at the same time)
+  asm volatile("" ::"s"(p), "s"(q), "s"(done));
+  pin_slots<SP>(pf);
   if (done) {
     if (fold && (blockIdx.x == 0) && (lane == 0))
       st->ctl[r ^ 1u] = st->ctl[r];
@@ -613,7 +682,7 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     load_rows<T, CPLX, B, E>(Vp, Vq, Vpi, Vqi, n, 0, 0, lane, vx, vy, vxi, vyi);
   PlainCtl<T> c = st->ctl[r];
   if (fold) {
-    const SlotTotals t = reduce_slots<B>(prev, (int)gridDim.x, stats, lane, lds);
+    const SlotTotals t = reduce_slots<B, SP>(prev, (int)gridDim.x, stats, lane, lds, pf);
     const Fold<T> f = fold_ctl<T, CPLX>(c, t, st->NRM_EXP);
     if ((k == 0) && (lane == 0))
       fold_commit<T>(st, r, t, f, st_idx - 1u, stp, swp, stats);
