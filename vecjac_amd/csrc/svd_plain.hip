@@ -649,18 +649,23 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   // loads from their use)
   constexpr int SP = (B >= 256) ? 2 : 4;
   const int np = (int)gridDim.x;
-  asm volatile("" ::"s"(row), "s"(st), "s"(prev), "s"(np)); // the arguments used first, read in one batch
+  // the arguments used before the pair's columns are requested, read in one batch
+  asm volatile("" ::"s"(row), "s"(st), "s"(prev), "s"(np), "s"(r), "s"(m), "s"(Gr), "s"(ldG));
+  if constexpr (CPLX)
+    asm volatile("" ::"s"(Gi));
   SlotPre<SP> pf;
   prefetch_slots<B, SP>(prev, np, lane, pf);
   const int p = (int)row[2 * k], q = (int)row[2 * k + 1];
   const int done = st->done;
-  // all three loads issued before the test below, and the slots received
-  // with them (they were requested at the same time)
-  asm volatile("" ::"s"(p), "s"(q), "s"(done));
+  PlainCtl<T> c = st->ctl[r];
+  const int nrm = st->NRM_EXP;
+  // the indices, the flag and the control state issued together before the
+  // test below, and the slots received with them (requested at the same time)
+  asm volatile("" ::"s"(p), "s"(q), "s"(done), "s"(c.MG), "s"(c.MV), "s"(c.sG), "s"(c.sV), "s"(nrm));
   pin_slots<SP>(pf);
   if (done) {
     if (fold && (blockIdx.x == 0) && (lane == 0))
-      st->ctl[r ^ 1u] = st->ctl[r];
+      st->ctl[r ^ 1u] = c;
     return;
   }
 
@@ -680,10 +685,9 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, 0, lane, xr, yr, xi, yi);
   if (vreg)
     load_rows<T, CPLX, B, E>(Vp, Vq, Vpi, Vqi, n, 0, 0, lane, vx, vy, vxi, vyi);
-  PlainCtl<T> c = st->ctl[r];
   if (fold) {
-    const SlotTotals t = reduce_slots<B, SP>(prev, (int)gridDim.x, stats, lane, lds, pf);
-    const Fold<T> f = fold_ctl<T, CPLX>(c, t, st->NRM_EXP);
+    const SlotTotals t = reduce_slots<B, SP>(prev, np, stats, lane, lds, pf);
+    const Fold<T> f = fold_ctl<T, CPLX>(c, t, nrm);
     if ((k == 0) && (lane == 0))
       fold_commit<T>(st, r, t, f, st_idx - 1u, stp, swp, stats);
     if (t.key != ~0ull)
