@@ -184,7 +184,10 @@ template <typename T> __device__ __forceinline__ void bcast_rows4(const DD<T> k,
 // Max of two values over the group.  Wave level: one swap32 scatters a to
 // the low half and b to the high half, the rows reduce with DPP, rows 0/1
 // and 2/3 combine through one swap16 and a last swap32 spreads a and b.
-template <typename T, int B>
+// The group reductions below write lds and read it back after a barrier;
+// PRE = false leaves out the barrier before the write, for an lds area
+// that no lane can still be reading.
+template <typename T, int B, bool PRE = true>
 __device__ __forceinline__ void red_max2(T &a, T &b, double *lds)
 {
   T lo = a, hi = b;
@@ -200,7 +203,8 @@ __device__ __forceinline__ void red_max2(T &a, T &b, double *lds)
     constexpr int W = B / 64;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    __syncthreads();
+    if constexpr (PRE)
+      __syncthreads();
     if (lane == 0) {
       lds[2 * w] = (double)a;
       lds[2 * w + 1] = (double)b;
@@ -237,7 +241,7 @@ template <typename A> __device__ __forceinline__ void red_dd2(DD<A> &q0, DD<A> &
 // scatters (q0, q2) and (q1, q3) over the halves, swap16 scatters the two
 // results over the rows (row r then holds a partial sum of q_r), the rows
 // reduce with DPP and the row totals are spread to every lane.
-template <typename A, int B>
+template <typename A, int B, bool PRE = true>
 __device__ __forceinline__ void red_dd4(DD<A> &q0, DD<A> &q1, DD<A> &q2, DD<A> &q3, double *lds)
 {
   const DD<A> k0 = dd_swapadd<true>(q0, q2);
@@ -248,7 +252,8 @@ __device__ __forceinline__ void red_dd4(DD<A> &q0, DD<A> &q1, DD<A> &q2, DD<A> &
     constexpr int W = B / 64;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    __syncthreads();
+    if constexpr (PRE)
+      __syncthreads();
     if (lane == 0) {
       double *o = lds + 8 * w;
       o[0] = (double)q0.s; o[1] = (double)q0.c;
@@ -271,7 +276,7 @@ __device__ __forceinline__ void red_dd4(DD<A> &q0, DD<A> &q1, DD<A> &q2, DD<A> &
 }
 
 // max over the group (used for the update maxima)
-template <typename T, int B>
+template <typename T, int B, bool PRE = true>
 __device__ __forceinline__ T red_max(T v, double *lds)
 {
   const T k = row_max(v);
@@ -283,7 +288,8 @@ __device__ __forceinline__ T red_max(T v, double *lds)
   if constexpr (B > 64) {
     constexpr int W = B / 64;
     const int w = threadIdx.x >> 6;
-    __syncthreads();
+    if constexpr (PRE)
+      __syncthreads();
     if ((threadIdx.x & 63) == 0)
       lds[w] = (double)v;
     __syncthreads();

@@ -448,8 +448,10 @@ __device__ __forceinline__ void fold_slot(SlotTotals &t, const PlainSlot *__rest
   }
 }
 
-// Slots [0, np): the first SP B from pf, the rest loaded here.
-template <int B, int SP>
+// Slots [0, np): the first SP B from pf, the rest loaded here.  POST =
+// false leaves out the barrier after the lds reads (the caller uses other
+// lds areas until its next barrier).
+template <int B, int SP, bool POST = true>
 __device__ __forceinline__ SlotTotals reduce_slots(const PlainSlot *__restrict__ sl, const int np, const int stats,
                                                    const int lane, double *lds, const SlotPre<SP> &pf)
 {
@@ -494,7 +496,8 @@ __device__ __forceinline__ SlotTotals reduce_slots(const PlainSlot *__restrict__
       const unsigned long long k2 = (unsigned long long)__double_as_longlong(u[5]);
       a.key = (k2 < a.key) ? k2 : a.key;
     }
-    __syncthreads(); // lds is reused by the step
+    if constexpr (POST)
+      __syncthreads(); // lds is reused by the step
     t = a;
   }
   return t;
@@ -633,7 +636,13 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   // The control state before this step is ctl[r] with, for st_idx > 0, the
   // slots of step st_idx - 1 (prev) folded in; workgroup 0 stores the fold
   // in ctl[r ^ 1].
-  __shared__ double lds[8 * (B / 64) + 2];
+  // lds areas of the group reductions: the fold of the slots (6 per wave),
+  // then, in the register form, the maxima (2), the sums (8) and the update
+  // maximum (1) each in an area of its own, so that none needs a barrier
+  // before its writes
+  constexpr int W64 = B / 64;
+  constexpr int L_MAX2 = 6 * W64, L_DD4 = 8 * W64, L_TG = 16 * W64;
+  __shared__ double lds[17 * W64 + 2];
   const int lane = threadIdx.x;
   const bool fold = (st_idx > 0u);
   // The pair and the flag are read together (one latency, not two).  A plain
@@ -686,7 +695,7 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   if (vreg)
     load_rows<T, CPLX, B, E>(Vp, Vq, Vpi, Vqi, n, 0, 0, lane, vx, vy, vxi, vyi);
   if (fold) {
-    const SlotTotals t = reduce_slots<B, SP>(prev, np, stats, lane, lds, pf);
+    const SlotTotals t = reduce_slots<B, SP, false>(prev, np, stats, lane, lds, pf);
     const Fold<T> f = fold_ctl<T, CPLX>(c, t, nrm);
     if ((k == 0) && (lane == 0))
       fold_commit<T>(st, r, t, f, st_idx - 1u, stp, swp, stats);
@@ -722,8 +731,28 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     const int nEV = (n + B - 1) / B;
     Step<T> Sg = S;
     Sg.vtrack = 0;
-    o = pair_step<T, CPLX, B, E>(m, 0, xr, yr, xi, yi, nullptr, nullptr, nullptr, nullptr, eSp, fSp, eSq, fSq, Sg, lds,
-                                 lane, D);
+    {
+      // pair_step with its reductions in their own lds areas
+      typedef typename Acc<T>::A A;
+      T mx, my;
+      pair_maxima<T, CPLX, E>(xr, yr, xi, yi, E, mx, my);
+      red_max2<T, B, false>(mx, my, lds + L_MAX2);
+      const int kx = max_exp(mx), ky = max_exp(my);
+      DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
+      pair_sums<T, CPLX, E>(xr, yr, xi, yi, E, kx, ky, sx, sy, dr, di);
+      red_dd4<A, B, false>(sx, sy, dr, di, lds + L_DD4);
+      const PairScal<T> P = pair_scal<T, CPLX>(mx, my, kx, ky, sx, sy, dr, di);
+      D = pair_decide<T, CPLX>(P, Sg.tol, Sg.gst, Sg.sec);
+      o = pair_apply<T, CPLX, B, E, false>(0, xr, yr, xi, yi, nullptr, nullptr, nullptr, nullptr, P, D, 0, eSp, fSp, eSq,
+                                           fSq, lds, lane);
+      if (!o.err) {
+        // (the update maximum and its check, as pair_apply makes them with
+        // its own reduction)
+        o.tG = red_max<T, B, false>(o.tG, lds + L_TG);
+        if (((o.code == 2) || (o.code == -2) || (o.code == 3) || (o.code == -3)) && !isfinite_(o.tG))
+          o.err = CPLX ? -630 : -580;
+      }
+    }
     if (!o.err && ((o.code != 1) || S.sG))
       store_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, 0, lane, xr, yr, xi, yi);
     if (!D.err) {
