@@ -383,6 +383,49 @@ __device__ __forceinline__ void scale_rows(const int s, T (&xr)[E], T (&yr)[E], 
   }
 }
 
+// A streamed step keeps the last KEEP chunks its first pass reads in LDS
+// (value j of a lane at [j][lane]), and its update pass, which visits them
+// first, takes them from there instead of reading them again: 32 KiB per
+// workgroup, within the LDS the streamed shapes' occupancy leaves free.
+template <typename T, bool CPLX, int B, int E> struct Keep {
+  static constexpr int V = E * (CPLX ? 4 : 2);                   // values per lane per chunk
+  static constexpr int N = (32768 / (int)sizeof(T)) / (V * B);    // chunks kept
+};
+
+template <typename T, bool CPLX, int B, int E>
+__device__ __forceinline__ void keep_rows(T *kb, const int lane, const T (&xr)[E], const T (&yr)[E], const T (&xi)[E],
+                                          const T (&yi)[E])
+{
+  constexpr int NV = CPLX ? 4 : 2; // values per row
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    kb[(NV * j + 0) * B + lane] = xr[j];
+    kb[(NV * j + 1) * B + lane] = yr[j];
+    if constexpr (CPLX) {
+      kb[(NV * j + 2) * B + lane] = xi[j];
+      kb[(NV * j + 3) * B + lane] = yi[j];
+    }
+  }
+}
+
+template <typename T, bool CPLX, int B, int E>
+__device__ __forceinline__ void kept_rows(const T *kb, const int lane, T (&xr)[E], T (&yr)[E], T (&xi)[E], T (&yi)[E])
+{
+  constexpr int NV = CPLX ? 4 : 2;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    xr[j] = kb[(NV * j + 0) * B + lane];
+    yr[j] = kb[(NV * j + 1) * B + lane];
+    if constexpr (CPLX) {
+      xi[j] = kb[(NV * j + 2) * B + lane];
+      yi[j] = kb[(NV * j + 3) * B + lane];
+    }
+    else {
+      xi[j] = yi[j] = (T)0;
+    }
+  }
+}
+
 // --------------------------------------------------------- fold of a step
 // Maxima, transformation count and first failure over the slots of one
 // step, returned to every lane (maxima, an integer sum and a minimum do not
@@ -643,6 +686,8 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   constexpr int W64 = B / 64;
   constexpr int L_MAX2 = 6 * W64, L_DD4 = 8 * W64, L_TG = 16 * W64;
   __shared__ double lds[17 * W64 + 2];
+  // the streamed form's kept chunks (each lane reads back only what it wrote)
+  __shared__ T kbuf[(STREAM && (Keep<T, CPLX, B, E>::N > 0)) ? Keep<T, CPLX, B, E>::N * Keep<T, CPLX, B, E>::V * B : 1];
   const int lane = threadIdx.x;
   const bool fold = (st_idx > 0u);
   // The pair and the flag are read together (one latency, not two).  A plain
@@ -788,8 +833,11 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
     const int kpx = pred ? (int)epx + S.sG : 0, kpy = pred ? (int)epy + S.sG : 0;
     T mx = (T)0, my = (T)0, nx = inf_<T>(), ny = inf_<T>(); // max and min nonzero |element|
     DD<A> sx{0, 0}, sy{0, 0}, dr{0, 0}, di{0, 0};
+    constexpr int KN = Keep<T, CPLX, B, E>::N, KV = Keep<T, CPLX, B, E>::V;
     for (int c = 0; c < nc; ++c) {
       load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
+      if (c >= nc - KN)
+        keep_rows<T, CPLX, B, E>(kbuf + (size_t)(nc - 1 - c) * KV * B, lane, xr, yr, xi, yi);
       T cx, cy;
       pair_maxima<T, CPLX, E>(xr, yr, xi, yi, E, cx, cy);
       mx = __builtin_fmax(mx, cx);
@@ -854,7 +902,10 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
                                            lds, lane, 0);
     for (int cc = 0; cc < (idle ? 0 : nc); ++cc) {
       const int c = nc - 1 - cc;
-      load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
+      if (cc < KN)
+        kept_rows<T, CPLX, B, E>(kbuf + (size_t)cc * KV * B, lane, xr, yr, xi, yi);
+      else
+        load_rows<T, CPLX, B, E>(Gp, Gq, Gpi, Gqi, m, c * E, S.sG, lane, xr, yr, xi, yi);
       // the V pair is updated with the first chunk visited (whose call
       // reports the V maxima of a preset V)
       o = pair_apply<T, CPLX, B, E, false>(cc ? 0 : n, xr, yr, xi, yi, Vp, Vq, Vpi, Vqi, P, D, S.vtrack, eSp, fSp, eSq,
