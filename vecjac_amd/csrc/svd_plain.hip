@@ -1175,9 +1175,11 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s, const PlainWo
       }
     }
   }
-  // Sweeps are enqueued one ahead of the convergence check: the host only
-  // reads the device flag of sweep k-1 after sweep k has been queued.  With a
-  // caller's workspace the flag of each sweep is read synchronously instead.
+  // Sweeps are enqueued ahead of the convergence check: the host reads the
+  // device flag of sweep k-1 after the first steps of sweep k have been
+  // queued (enough work to cover the wait and the queueing of the rest), and
+  // queues the rest of sweep k only when the run goes on.  With a caller's
+  // workspace the flag of each sweep is read synchronously instead.
   PlainFlags *hf = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   if (!work) {
@@ -1195,13 +1197,31 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s, const PlainWo
   // ctl[r] is the control state before the next launch; each launch that
   // folds a step moves it to ctl[r ^ 1]
   unsigned r = 0;
+  const unsigned head = std::min(a.stp, 128u); // steps queued before the previous sweep's flag is read
   for (unsigned sw = 0; !rc && sw < a.swp; ++sw) {
+    bool ended = false;
     for (unsigned k = 0; k < a.stp; ++k) {
+      if (!work && (sw >= 1) && (k == head)) {
+        ++launched; // (partly queued)
+        if (hipEventSynchronize(ev[(sw - 1) & 1u]) != hipSuccess) {
+          rc = set_error(VJ_EHIP, "event sync failed");
+          break;
+        }
+        if (hf[(sw - 1) & 1u].done) {
+          // the queued steps return at once; their launches that fold keep
+          // the control state's parity (done launches copy ctl[r] forward)
+          ended = true;
+          break;
+        }
+        --launched;
+      }
       step(a, st, r, slots + ((k + 1u) & 1u) * np, slots + (k & 1u) * np, a.js + (size_t)k * a.n, k, tol, gst, vrec,
            s);
       if (k)
         r ^= 1u;
     }
+    if (rc || ended)
+      break;
     hipLaunchKernelGGL((k_fold<T, CPLX>), dim3(1), dim3(256), 0, s, st, slots + ((a.stp - 1u) & 1u) * np, (int)np, r,
                        a.stp, a.swp, a.stats);
     r ^= 1u;
@@ -1226,7 +1246,8 @@ int plain_svd(const PlainArgs<T> &a, PlainRun *run, hipStream_t s, const PlainWo
       rc = set_error(VJ_EHIP, "flag copy failed");
       break;
     }
-    if (sw >= 1) {
+    if ((sw >= 1) && (a.stp <= head)) {
+      // a sweep of at most `head` steps: its flag read after the whole sweep
       if (hipEventSynchronize(ev[(sw - 1) & 1u]) != hipSuccess) {
         rc = set_error(VJ_EHIP, "event sync failed");
         break;
