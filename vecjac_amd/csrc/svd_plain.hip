@@ -383,7 +383,7 @@ __device__ __forceinline__ void scale_rows(const int s, T (&xr)[E], T (&yr)[E], 
   }
 }
 
-// A streamed step keeps the last KEEP chunks its first pass reads in LDS
+// A streamed step keeps the last Keep::N chunks its first pass reads in LDS
 // (value j of a lane at [j][lane]), and its update pass, which visits them
 // first, takes them from there instead of reading them again: 32 KiB per
 // workgroup, within the LDS the streamed shapes' occupancy leaves free.
@@ -699,8 +699,8 @@ __global__ __launch_bounds__(B) void k_step(const int m, const int n, T *__restr
   // the pair, so their latency overlaps that of the pair's indices and then
   // of its columns
   // (unconditionally: the first step of a sweep reads the other half of the
-  // slot buffer, allocated and unused, so that no branch separates the
-  // loads from their use)
+  // slot buffer, allocated, whose contents it does not use, so that no
+  // branch separates the loads from their use)
   constexpr int SP = (B >= 256) ? 2 : 4;
   const int np = (int)gridDim.x;
   // the arguments used before the pair's columns are requested, read in one batch
