@@ -248,8 +248,9 @@ vjh_int vjh_dvjsvd_noalloc(vjh_nat m, vjh_nat n, double *G, vjh_nat ldG, double 
 /* Plan for a batch of nb independent SVDs solved in three persistent
  * launches, one per class: matrices with m above the split (128, complex
  * 64, by default), in the wave form; then the other ("short") ones in the
- * lane-group form, first those whose G and V exceed the LDS budget (48 KiB
- * by default), on global memory, then those within it, resident in LDS.
+ * lane-group form, first those whose G exceeds the LDS budget (48 KiB by
+ * default), on global memory, then those within it, with G resident in LDS
+ * (V in device memory).
  * kind: 0 = FP64 real, 1 = FP64 complex (split re/im), 2 = FP32 real,
  * 3 = FP32 complex.  Host descriptors: m[b], n[b] (n even, 2 <= n <= 128,
  * n <= m <= 1024), element offsets offG[b] of G_b (ld m[b]), offV[b] of V_b (ld n[b])

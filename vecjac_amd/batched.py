@@ -1,6 +1,6 @@
 """Batched Jacobi SVD of many uneven matrices (one persistent launch per class
 of matrices per run: long columns; short columns on global memory; short
-columns whose G and V stay resident in LDS).
+columns whose G stays resident in LDS).
 
 Layout (device memory, one buffer per quantity): matrix b has its m_b x n_b
 G (A on entry, U on exit) at element offset offG[b], column-major with

@@ -21,10 +21,10 @@ namespace {
 // A run is three launches, one per class of matrices, each class in its own
 // stretch of the processing order with its own work counter:
 //   0: long columns (m > short_rows), the wave form;
-//   1: short columns whose G and V exceed the LDS budget, the lane-group form
-//      on global memory;
-//   2: short columns whose G and V fit the budget, the lane-group form
-//      resident in LDS.
+//   1: short columns whose G exceeds the LDS budget, the lane-group form on
+//      global memory;
+//   2: short columns whose G fits the budget, the lane-group form with G
+//      resident in LDS (V in device memory).
 constexpr int NCLASS = 3;
 
 struct BPlan {
@@ -36,7 +36,7 @@ struct BPlan {
   int waves_c[NCLASS] = {4, 4, 4}; // waves per workgroup of each launch
   int lg_c[NCLASS] = {4, 4, 4};    // lane-group launches: 2^lg lanes per pair
   int split_rows = 128;     // matrices with more rows take the wave form
-  long long lds_budget = 0; // bytes of G and V up to which a short matrix is LDS-resident
+  long long lds_budget = 0; // bytes of G up to which a short matrix is LDS-resident
   int lds_bytes = 0;       // dynamic LDS per workgroup of the LDS launch (max over its class)
   long long totG = 0;      // elements of the packed G (max over b of offG + m n)
   unsigned swp = 999u;
@@ -58,8 +58,8 @@ __global__ void k_zero_u32(unsigned *p)
     p[i] = 0u;
 }
 
-// Default LDS budget: G and V of up to 48 KiB, so that three four-wave
-// workgroups (the register limit of the lane-group form) fit a CU.
+// Default LDS budget: G of up to 48 KiB, so that three four-wave workgroups
+// (the register limit of the lane-group form) fit a CU.
 constexpr long long DEFAULT_LDS_BUDGET = 48 * 1024;
 
 int tsize(int kind) { return (kind <= 1) ? 8 : 4; }

@@ -18,7 +18,7 @@ template <typename T> struct BatchArgs {
   unsigned long long *xfer;       // per matrix: elements of G and V read or written in memory (may be null)
   unsigned *queue;                // work counter, zero on entry
   unsigned swp;                   // sweep limit
-  int lds_bytes;                  // LDS-resident form: bytes of G and V per workgroup (0: global)
+  int lds_bytes;                  // LDS-resident form: bytes of G per workgroup (0: global)
   int group_lg;                   // lane-group form: 2^group_lg lanes per pair (4, 3 or 2)
   int ord;                        // ordering of the pairs: pjs id 4 (modulus) or 6 (block modulus)
   int vpreset;                    // V holds the starting matrices (else V := I)
@@ -39,16 +39,16 @@ constexpr int group_max_rows(bool cplx, int lg)
 
 // One launch over a.nb matrices of a.order: grp selects the short-column
 // form (lane groups of 16, m <= 128 real / 64 complex), else the wave form;
-// a.lds_bytes > 0 (short-column form only) keeps each matrix's G and V in
-// LDS for all its sweeps.
+// a.lds_bytes > 0 (short-column form only) keeps each matrix's G in LDS for
+// all its sweeps.
 template <typename T, bool CPLX>
 int batched_svd(const BatchArgs<T> &a, int waves, int max_m, int grid_cap, bool grp, hipStream_t s);
 
-// LDS bytes a matrix needs in the LDS-resident form: G (m x n) and V (n x n),
-// both planes for complex data.
+// LDS bytes a matrix needs in the LDS-resident form: G (m x n), both planes
+// for complex data (V stays in device memory).
 inline long long lds_matrix_bytes(long long m, long long n, bool cplx, int tsize)
 {
-  return (m + n) * n * (long long)tsize * (cplx ? 2 : 1);
+  return m * n * (long long)tsize * (cplx ? 2 : 1);
 }
 
 // Largest dynamic LDS a workgroup of the LDS-resident form may take (the

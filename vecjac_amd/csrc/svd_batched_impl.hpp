@@ -1175,10 +1175,12 @@ template <typename T, bool CPLX, int E, int W, bool PF, int LG>
 __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                            const int ord, const unsigned st, const int wave, const int lane,
                                            const Step<T> &S, T &wG, T &wV, unsigned long long &wcnt, int &werr,
-                                           unsigned long long &wev, unsigned long long *wxf)
+                                           unsigned long long &wev, unsigned long long *wxf, const bool gmem)
 {
   typedef typename Acc<T>::A A;
   constexpr int lg = LG, LP = 1 << LG, P = 64 >> LG; // lanes per group, groups per wave
+  // (wxf counts the elements moved through device memory: G's columns only
+  // when G is there, gmem, V's always)
   constexpr unsigned long long PL = CPLX ? 2ull : 1ull; // planes
   const int npairs = n >> 1;
   if (wave >= npairs)
@@ -1186,7 +1188,7 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
   const int cnt = (npairs - wave + W - 1) / W; // pairs of this wave
   wev += (unsigned long long)cnt;
   if (wxf)
-    *wxf += (unsigned long long)cnt * 2ull * PL * (unsigned long long)(m + n);
+    *wxf += (unsigned long long)cnt * 2ull * PL * (unsigned long long)((gmem ? m : 0) + n);
   const int nit = (cnt + P - 1) >> (6 - lg);
   const int g = lane >> lg, t = lane & (LP - 1);
   const int nE = (m + LP - 1) >> lg, nEV = (n + LP - 1) >> lg;
@@ -1324,7 +1326,7 @@ __device__ __forceinline__ void group_step(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T 
       store_cols<T, CPLX, E>(Vr, Vi, n, nEV, dp, dq, t, LP, vx, vy, vxi, vyi);
     if (wxf)
       *wxf += 2ull * PL *
-              ((unsigned long long)__popcll(__ballot(stG && (t == 0))) * (unsigned long long)m +
+              ((unsigned long long)__popcll(__ballot(gmem && stG && (t == 0))) * (unsigned long long)m +
                (unsigned long long)__popcll(__ballot(stV && (t == 0))) * (unsigned long long)n);
     // (e,f) of the pair as pair_apply leaves them: exchanged for -1 / -3
     if (valid && !err && (t == 0)) {
@@ -1367,17 +1369,18 @@ template <typename T, bool CPLX, int E, int W, int LG>
 __device__ __forceinline__ void group_dispatch(T *Gr, T *Gi, T *Vr, T *Vi, T *eS, T *fS, const int m, const int n,
                                                const int ord, const unsigned st, const int wave, const int lane,
                                                const Step<T> &S, T &wG, T &wV, unsigned long long &wcnt, int &werr,
-                                               unsigned long long &wev, unsigned long long *wxf)
+                                               unsigned long long &wev, unsigned long long *wxf, const bool gmem)
 {
   constexpr int E64 = 64 >> LG; // chunks of 2^LG rows for 64 rows
   if ((E < 2) || CPLX || (m <= 64)) {
     group_step<T, CPLX, E64, W, GroupShape<T, CPLX, E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, ord, st, wave, lane, S,
-                                                                  wG, wV, wcnt, werr, wev, wxf);
+                                                                  wG, wV, wcnt, werr, wev, wxf, gmem);
     return;
   }
   if constexpr ((E >= 2) && !CPLX && (2 * E64 <= 16))
     group_step<T, CPLX, 2 * E64, W, GroupShape<T, CPLX, 2 * E64>::PF, LG>(Gr, Gi, Vr, Vi, eS, fS, m, n, ord, st, wave,
-                                                                          lane, S, wG, wV, wcnt, werr, wev, wxf);
+                                                                          lane, S, wG, wV, wcnt, werr, wev, wxf,
+                                                                          gmem);
 }
 
 // Longer columns: a pair per 64 lanes, two pairs per wave, with the register
@@ -1463,16 +1466,19 @@ template <typename T, bool CPLX, int E> struct BsvdMinWaves {
 };
 
 // LDS-resident form (LDS = true, lane-group steps only): the setup scan
-// copies the matrix's G into dynamic LDS, V := I is formed there, every step
-// of every sweep reads and writes those copies, and G (U after the
-// epilogue) and V are copied out once at the end, so a step's latency is
-// that of LDS rather than of a round trip through the caches.
+// copies the matrix's G into dynamic LDS, every step of every sweep reads
+// and writes that copy, and G (U after the epilogue) is copied out once at
+// the end, so a step's G latency is that of LDS rather than of a round trip
+// through the caches.  V (n x n, at most a third of the matrix's elements
+// for m >= 2n) stays in device memory, where it is read and written through
+// the caches: the LDS a matrix takes is that of G alone, and more matrices
+// are resident per CU.
 // LG (lane-group form): 2^LG lanes per pair.
 template <typename T, bool CPLX, int E, int W, bool GRP, bool LDS, int LG>
 __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_bsvd(const BatchArgs<T> a)
 {
   static_assert(!LDS || GRP, "the LDS-resident form uses the lane-group steps");
-  extern __shared__ double s_dyn[];            // LDS form: G (re, im), then V (re, im)
+  extern __shared__ double s_dyn[];            // LDS form: G (re, im)
   __shared__ int s_b;
   __shared__ T s_eS[128], s_fS[128];           // the matrix's (e,f) column norms (n <= 128)
   __shared__ int s_mod[128];                   // step at which each column was last modified
@@ -1499,11 +1505,10 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     T *const gVr = a.Vr + a.offV[b];
     T *const gVi = CPLX ? a.Vi + a.offV[b] : nullptr;
     T *const lG = (T *)s_dyn;
-    T *const lV = lG + (CPLX ? 2 : 1) * m * n;
     T *const Gr = LDS ? lG : gGr;
     T *const Gi = CPLX ? (LDS ? lG + m * n : gGi) : nullptr;
-    T *const Vr = LDS ? lV : gVr;
-    T *const Vi = CPLX ? (LDS ? lV + n * n : gVi) : nullptr;
+    T *const Vr = gVr;
+    T *const Vi = CPLX ? gVi : nullptr;
     T *const eS = a.eS + a.offS[b];
     T *const fS = a.fS + a.offS[b];
     const unsigned stp = (unsigned)n; // steps per sweep (pjs id 4 or 6)
@@ -1546,20 +1551,16 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
     for (int w = 1; w < W; ++w)
       MG = xmax(MG, (T)s_max[0][w][0]);
     __syncthreads();
-    // a preset V (dvjsvd.c:89-96): its max |.| (copied into LDS with it)
+    // a preset V (dvjsvd.c:89-96): its max |.|
     T MV0 = (T)1;
     if (a.vpreset) {
       T mv = (T)0;
       for (int t = threadIdx.x; t < n * n; t += 64 * W) {
         const T v = gVr[t];
         mv = xmax(xmin(vabs(v), inf_<T>()), mv);
-        if constexpr (LDS)
-          Vr[t] = v;
         if constexpr (CPLX) {
           const T vi = gVi[t];
           mv = xmax(xmin(vabs(vi), inf_<T>()), mv);
-          if constexpr (LDS)
-            Vi[t] = vi;
         }
       }
 #pragma unroll
@@ -1649,7 +1650,7 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
           int werr = 0;
           if constexpr (GRP)
             group_dispatch<T, CPLX, E, W, LG>(Gr, Gi, Vr, Vi, s_eS, s_fS, m, n, a.ord, st, wave, lane, S, wG, wV,
-                                              wcnt, werr, wev, LDS ? nullptr : &wxf);
+                                              wcnt, werr, wev, &wxf, !LDS);
           else {
             if constexpr (!CPLX) {
               if (blk) {
@@ -1743,18 +1744,13 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       }
     }
     if constexpr (LDS) {
-      // G (U after the epilogue, else the working G) and V out of LDS
+      // G (U after the epilogue, else the working G) out of LDS
       __syncthreads();
       if (norms) {
         for (long long t = threadIdx.x; t < (long long)m * n; t += 64 * W) {
           gGr[t] = Gr[t];
           if constexpr (CPLX)
             gGi[t] = Gi[t];
-        }
-        for (int t = threadIdx.x; t < n * n; t += 64 * W) {
-          gVr[t] = Vr[t];
-          if constexpr (CPLX)
-            gVi[t] = Vi[t];
         }
       }
     }
@@ -1775,9 +1771,11 @@ __global__ __launch_bounds__(64 * W, (BsvdMinWaves<T, CPLX, E>::value)) void k_b
       if (a.evals)
         a.evals[b] = s_ev;
       if (a.xfer) {
-        // the LDS-resident form moves G in, and U (or G) and V out, once
+        // the LDS-resident form moves G in and U (or G) out once, and V
+        // (formed, and read and written by the steps) through device memory
         const unsigned long long pl = CPLX ? 2ull : 1ull;
-        a.xfer[b] = LDS ? pl * (unsigned long long)n * (unsigned long long)(norms ? (2 * m + n) : m) : (s_xf + pl * xset);
+        a.xfer[b] = LDS ? s_xf + pl * (unsigned long long)n * (unsigned long long)(norms ? (2 * m + n) : m)
+                        : (s_xf + pl * xset);
       }
     }
   }
