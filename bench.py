@@ -178,8 +178,8 @@ def _bsvd_alg_bytes(m, n, evals, transforms, w):
     (w 2m) and a transformed pair writes them and reads/writes its two V
     columns (w (2m + 4n)).  evals counts the pair evaluations the run made (a
     pair whose columns did not change since its previous evaluation is not
-    evaluated again).  (The LDS-resident form moves these bytes between LDS
-    and registers.)"""
+    evaluated again).  (The LDS-resident form moves the G bytes between LDS
+    and registers, and the V bytes through the caches.)"""
     m = m.astype(np.float64)
     n = n.astype(np.float64)
     return float(np.sum(evals.astype(np.float64) * w * 2 * m + transforms.astype(np.float64) * w * (2 * m + 4 * n)))
@@ -730,9 +730,10 @@ def bench_grad(args):
         "roofline": {"bound": "hbm", "achieved": fwd_bytes / (fk_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": fwd_bytes / (fk_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_bsvd (forward)", "kernel_avg_ms": fk_ms, "alg_bytes_per_launch": fwd_bytes,
-                     "note": "algorithmic bytes are the per-step column traffic; in the LDS-resident class it moves "
-                             "between LDS and registers and HBM sees G in and U, V out once (about 0.17 GB per run, "
-                             "profiles/k_bsvd_fp64_4096x64x32.md): the launch is bound by VALU instruction issue",
+                     "note": "algorithmic bytes are the per-step column traffic; in the LDS-resident class the G "
+                             "columns move between LDS and registers and the V columns through the caches, and HBM "
+                             "sees about 2.7 GB per run (profiles/k_bsvd_fp64_4096x64x32.md): the launch is bound by "
+                             "VALU instruction issue",
                      "backward": {"kernel": "k_svd_bwd_stg" if (m * n <= 2048 and n * n <= 2048) else "k_svd_bwd",
                                   "kernel_avg_ms": b_ms, "alg_bytes_per_launch": bwd_bytes,
                                   "achieved": bwd_bytes / (b_ms / 1e3) / 1e9,
