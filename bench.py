@@ -252,8 +252,8 @@ def bench_bsvd(args, kind: str = "d"):
     step_bytes = _bsvd_alg_bytes(m, npad, r["evals"], tr, w)
     # the G and V elements this design actually moves to and from memory
     # (every column load and store of the steps or block rounds, the set-up
-    # scan, V := I, the scaling and the epilogue; G in and U, V out for a
-    # matrix held in LDS), counted by the kernel (vjh_bplan_traffic): below
+    # scan, V := I, the scaling and the epilogue; G in and U out once for a
+    # matrix whose G is held in LDS), counted by the kernel (vjh_bplan_traffic): below
     # the algorithmic bytes where block rounds and LDS residence reuse columns
     # on chip
     moved = float(np.sum(r["xfer"].astype(np.float64))) * w

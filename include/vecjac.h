@@ -291,7 +291,8 @@ vjh_int vjh_bplan_evals(const void *plan, uint64_t *evals, void *stream);
  * solve in the plan's last run read from or wrote to device memory
  * (uint64_t elems[nb]): the column loads and stores of every step, the
  * set-up scan, V := I, the initial scaling and the epilogue; for a matrix
- * held in LDS, G read once and U and V written once.  Synchronises `stream`. */
+ * whose G is held in LDS, G read once and U written once, and V's columns
+ * as above.  Synchronises `stream`. */
 vjh_int vjh_bplan_traffic(const void *plan, uint64_t *elems, void *stream);
 /* Ordering of the pairs in the plan's runs: 6 (default), the block modulus
  * ordering of pjs(6, n), whose long-column launch keeps the four columns of
